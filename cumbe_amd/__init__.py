@@ -1,0 +1,219 @@
+"""Python access to libmbe.so, the MI355X maximal biclique enumeration engine.
+
+This is a thin ctypes mirror of the C ABI in include/mbe.h. The reference's
+only interface is its command line (``mbe <graph.bi> <cuMBE|noRS|noES|noWS>``,
+src/main.cu:121-122); the same entry point is the ``mbe`` binary built next to
+the library, and :func:`solve` is its in-process form.
+
+There is no CPU fallback: if libmbe.so is missing, importing the solver
+functions raises, and a solve without a HIP device returns MBE_EHIP.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from typing import Iterable, List, Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmbe.so")
+CLI_PATH = os.path.join(_HERE, "mbe")
+
+VARIANTS = {"cuMBE": 0, "noRS": 1, "noES": 2, "noWS": 3}
+
+STATUS = {
+    0: "ok", -1: "invalid argument", -2: "out of memory", -3: "HIP error",
+    -4: "timeout", -5: "over admission budget", -6: "I/O error", -7: "busy",
+    -8: "not ready",
+}
+
+
+class MbeError(RuntimeError):
+    def __init__(self, status: int, detail: str = ""):
+        self.status = status
+        super().__init__(f"{STATUS.get(status, status)}: {detail}")
+
+
+class Tuning(C.Structure):
+    _fields_ = [
+        ("workers", C.c_uint32),
+        ("waves_per_cu", C.c_uint32),
+        ("lds_words", C.c_uint32),
+        ("arena_words", C.c_uint32),
+        ("list_cutoff", C.c_uint32),
+        ("donate_depth", C.c_uint32),
+        ("side", C.c_int32),
+        ("flags", C.c_uint32),
+    ]
+
+
+class Result(C.Structure):
+    _fields_ = [
+        ("count", C.c_uint64),
+        ("hash", C.c_uint64),
+        ("kernel_ms", C.c_double),
+        ("total_ms", C.c_double),
+        ("nodes", C.c_uint64),
+        ("donated", C.c_uint64),
+        ("launches", C.c_uint32),
+        ("workers", C.c_uint32),
+    ]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load libmbe.so (built in-tree by __graft_entry__.build())."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    G = C.c_void_p
+    L.mbe_status_str.restype = C.c_char_p
+    L.mbe_last_error.restype = C.c_char_p
+    L.mbe_graph_load_bi.argtypes = [C.c_char_p, P(G)]
+    L.mbe_graph_write_bi.argtypes = [G, C.c_char_p]
+    L.mbe_graph_from_edge_list.argtypes = [C.c_char_p, C.c_int, C.c_int, P(G)]
+    L.mbe_graph_from_csr.argtypes = [C.c_uint32, C.c_uint32, P(C.c_uint32), P(C.c_uint32), P(G)]
+    L.mbe_graph_info.argtypes = [G, P(C.c_uint32), P(C.c_uint32), P(C.c_uint64)]
+    L.mbe_graph_csr.argtypes = [G, P(P(C.c_uint32)), P(P(C.c_uint32))]
+    L.mbe_graph_free.argtypes = [G]
+    L.mbe_graph_free.restype = None
+    L.mbe_gen_chung_lu.argtypes = [C.c_uint32, C.c_uint32, C.c_uint64, C.c_double, C.c_double,
+                                   C.c_uint32, C.c_uint64, P(G)]
+    L.mbe_gen_er_planted.argtypes = [C.c_uint32, C.c_uint32, C.c_double, C.c_uint32, C.c_uint32,
+                                     C.c_uint32, C.c_double, C.c_uint64, P(G)]
+    L.mbe_gen_batch_graph.argtypes = [C.c_uint64, P(G)]
+    L.mbe_solve.argtypes = [G, C.c_int, P(Tuning), P(Result)]
+    L.mbe_batch_solve.argtypes = [P(G), C.c_size_t, C.c_int, P(Tuning), P(Result), P(Result)]
+    for name in ("mbe_graph_load_bi", "mbe_graph_write_bi", "mbe_graph_from_edge_list",
+                 "mbe_graph_from_csr", "mbe_graph_info", "mbe_graph_csr", "mbe_gen_chung_lu",
+                 "mbe_gen_er_planted", "mbe_gen_batch_graph", "mbe_solve", "mbe_batch_solve"):
+        getattr(L, name).restype = C.c_int
+    _lib = L
+    return L
+
+
+def _check(status: int) -> None:
+    if status != 0:
+        raise MbeError(status, lib().mbe_last_error().decode(errors="replace"))
+
+
+class Graph:
+    """A host CSR over U (rows of the .bi file), sorted and deduplicated."""
+
+    def __init__(self, handle: int):
+        self._h = C.c_void_p(handle)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib is not None:
+            _lib.mbe_graph_free(h)
+            self._h = None
+
+    @property
+    def handle(self) -> C.c_void_p:
+        return self._h
+
+    @classmethod
+    def _make(cls, fn, *args) -> "Graph":
+        h = C.c_void_p()
+        _check(fn(*args, C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def load_bi(cls, path: str) -> "Graph":
+        return cls._make(lib().mbe_graph_load_bi, path.encode())
+
+    @classmethod
+    def from_edge_list(cls, path: str, skip_words: int = 0, skip_per_edge: int = 0) -> "Graph":
+        return cls._make(lib().mbe_graph_from_edge_list, path.encode(), skip_words, skip_per_edge)
+
+    @classmethod
+    def from_csr(cls, n_u: int, n_v: int, off, adj) -> "Graph":
+        off = np.ascontiguousarray(off, dtype=np.uint32)
+        adj = np.ascontiguousarray(adj, dtype=np.uint32)
+        if len(off) != n_u + 1:
+            raise ValueError("offsets must have n_u + 1 entries")
+        P = C.POINTER(C.c_uint32)
+        adj_p = adj.ctypes.data_as(P) if adj.size else C.cast(C.c_void_p(0), P)
+        return cls._make(lib().mbe_graph_from_csr, n_u, n_v, off.ctypes.data_as(P), adj_p)
+
+    @classmethod
+    def from_edges(cls, n_u: int, n_v: int, edges: Iterable[Sequence[int]]) -> "Graph":
+        e = np.asarray(list(edges), dtype=np.int64).reshape(-1, 2)
+        order = np.lexsort((e[:, 1], e[:, 0])) if len(e) else np.zeros(0, dtype=np.int64)
+        e = e[order]
+        off = np.zeros(n_u + 1, dtype=np.uint32)
+        if len(e):
+            np.add.at(off, e[:, 0] + 1, 1)
+        off = np.cumsum(off).astype(np.uint32)
+        return cls.from_csr(n_u, n_v, off, e[:, 1].astype(np.uint32))
+
+    @classmethod
+    def chung_lu(cls, n_u, n_v, m, gamma_u, gamma_v, max_deg, seed) -> "Graph":
+        return cls._make(lib().mbe_gen_chung_lu, n_u, n_v, m, gamma_u, gamma_v, max_deg, seed)
+
+    @classmethod
+    def er_planted(cls, n_u, n_v, p, n_planted, lo, hi, top_frac, seed) -> "Graph":
+        return cls._make(lib().mbe_gen_er_planted, n_u, n_v, p, n_planted, lo, hi, top_frac, seed)
+
+    @classmethod
+    def batch_graph(cls, seed: int) -> "Graph":
+        return cls._make(lib().mbe_gen_batch_graph, seed)
+
+    def write_bi(self, path: str) -> None:
+        _check(lib().mbe_graph_write_bi(self._h, path.encode()))
+
+    def info(self):
+        nu, nv, ne = C.c_uint32(), C.c_uint32(), C.c_uint64()
+        _check(lib().mbe_graph_info(self._h, C.byref(nu), C.byref(nv), C.byref(ne)))
+        return nu.value, nv.value, ne.value
+
+    def csr(self):
+        """(n_u, n_v, off, adj) as numpy copies."""
+        nu, nv, ne = self.info()
+        off_p = C.POINTER(C.c_uint32)()
+        adj_p = C.POINTER(C.c_uint32)()
+        _check(lib().mbe_graph_csr(self._h, C.byref(off_p), C.byref(adj_p)))
+        off = np.ctypeslib.as_array(off_p, shape=(nu + 1,)).copy()
+        adj = np.ctypeslib.as_array(adj_p, shape=(max(ne, 1),))[:ne].copy() if ne else np.zeros(0, np.uint32)
+        return nu, nv, off, adj
+
+
+def _tuning(tuning: Optional[dict]) -> Optional[Tuning]:
+    if not tuning:
+        return None
+    t = Tuning()
+    t.side = -1
+    for k, v in tuning.items():
+        setattr(t, k, v)
+    return t
+
+
+def solve(graph: Graph, variant: str = "cuMBE", tuning: Optional[dict] = None) -> Result:
+    """Enumerate all maximal bicliques of `graph` on the GPU (one launch per graph)."""
+    r = Result()
+    t = _tuning(tuning)
+    _check(lib().mbe_solve(graph.handle, VARIANTS[variant], C.byref(t) if t else None, C.byref(r)))
+    return r
+
+
+def batch_solve(graphs: List[Graph], variant: str = "cuMBE", tuning: Optional[dict] = None):
+    """All graphs in one persistent launch; returns (per-graph results, total)."""
+    n = len(graphs)
+    arr = (C.c_void_p * max(n, 1))(*[g.handle.value for g in graphs])
+    per = (Result * max(n, 1))()
+    tot = Result()
+    t = _tuning(tuning)
+    _check(lib().mbe_batch_solve(arr, n, VARIANTS[variant], C.byref(t) if t else None, per, C.byref(tot)))
+    return list(per)[:n], tot

@@ -1,0 +1,68 @@
+// Structures shared by the host engine (api.cpp) and the gfx950 kernels
+// (mbe_kernel.hip). Everything here is plain data laid out for HBM.
+#pragma once
+#include <cstddef>
+#include <cstdint>
+
+namespace mbe {
+
+// One prepared graph resident in device memory. R = enumerated side, ids are
+// root-order ranks; L keeps file ids. Both CSRs have sorted lists.
+struct DevGraph {
+    const uint32_t* offR;   // nR + 1
+    const uint32_t* adjR;   // L ids
+    const uint32_t* offL;   // nL + 1
+    const uint32_t* adjL;   // R ranks
+    const uint64_t* hR;     // hash term per rank (file id of the R side)
+    const uint64_t* hL;     // hash term per L id
+    uint32_t nR, nL;
+    uint32_t n_roots;       // ranks [0, n_roots) have nonzero degree
+    uint32_t r_is_u;        // 1 when R holds the file's rows
+    uint64_t task_base;     // first global task id of this graph
+};
+
+// Counters the kernel accumulates for profiling (per launch, summed).
+enum KStat : int {
+    kStatNodes = 0,      // enumeration-tree nodes processed (one per chosen x)
+    kStatRoots = 1,      // root tasks completed
+    kStatDonated = 2,    // subtree tasks pushed to the deque
+    kStatStolen = 3,     // subtree tasks taken from the deque
+    kStatOverflow = 4,   // root tasks deferred for a larger arena
+    kStatBuildWords = 5, // adjacency entries read while building local problems
+    kStatCount = 8
+};
+
+// Deque slot header for donated subtrees (see mbe_kernel.hip).
+constexpr uint32_t kSlotHeaderWords = 8;
+
+struct KParams {
+    const DevGraph* graphs;
+    uint32_t n_graphs;
+    uint32_t variant;           // mbe_variant
+    uint64_t n_tasks;           // tasks in this launch
+    const uint64_t* task_list;  // nullable: explicit global task ids
+    unsigned long long* cursor; // next task index (atomic)
+    unsigned long long* results;// 2 per graph: count, hash
+    unsigned long long* stats;  // kStatCount
+    // per-worker resources
+    uint64_t* arena;            // workers * arena_words
+    uint64_t arena_words;
+    uint64_t* mark;             // workers * mark_words (bitmap over R)
+    uint64_t mark_words;
+    uint32_t* cpos;             // workers * cpos_len (R rank -> local index)
+    uint64_t cpos_len;
+    uint32_t lds_words;         // dynamic LDS per worker (u64 words)
+    uint32_t list_cutoff;       // P/Q popcount at or below which rows are scanned
+    // roots that did not fit the arena
+    unsigned long long* overflow_count;
+    uint64_t* overflow_list;
+    uint64_t overflow_cap;
+    // work-stealing deque (donated subtrees)
+    uint64_t* slots;            // n_slots * slot_words
+    uint64_t slot_words;
+    uint32_t n_slots;
+    uint32_t donate_depth;
+    unsigned long long* dq;     // [0] head, [1] tail, [2] idle, [3] pending, [4..] seq per slot
+};
+
+}  // namespace mbe

@@ -1,0 +1,120 @@
+/*
+ * mbe.h -- C ABI of the MI355X maximal biclique enumeration engine (libmbe.so).
+ *
+ * Plain pointers and sizes only; no torch or HIP types cross this boundary.
+ * Status codes are returned, never thrown; the library never aborts on bad
+ * input. The caller owns every input and result buffer; the library owns
+ * every device buffer.
+ *
+ * Reference interfaces replaced (cuMBE, paths relative to its repository):
+ *   mbe_graph_load_bi ........ .bi reader in main, src/main.cu:38-48
+ *   mbe_graph_write_bi ....... .bi writer, data/gen_bi.cpp:72-79
+ *   mbe_graph_from_edge_list . edge list -> CSR relabel/dedupe, data/gen_bi.cpp:34-51
+ *   mbe_solve ................ side swap + CUDA_TRANSPOSE + root sort + the
+ *                              cooperative CUDA_MBE_<variant> launch and the
+ *                              num_mb readback, src/main.cu:65-207
+ *   mbe_variant .............. the argv[2] algorithm switch, src/main.cu:121-122
+ *   mbe_result.count ......... the "maximal bicliques: N" line, src/main.cu:207
+ *   mbe_result.kernel_ms ..... cudaEvent span around the MBE kernel,
+ *                              src/main.cu:185-204
+ */
+#ifndef MBE_H
+#define MBE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    MBE_OK = 0,
+    MBE_EINVAL = -1,       /* bad argument or malformed input */
+    MBE_ENOMEM = -2,       /* host or device allocation failed */
+    MBE_EHIP = -3,         /* a HIP call failed; see mbe_last_hip_error() */
+    MBE_ETIMEOUT = -4,     /* a watchdog or poll deadline expired */
+    MBE_EOVERBUDGET = -5,  /* request exceeds the admission budget */
+    MBE_EIO = -6,          /* file could not be read or written */
+    MBE_EBUSY = -7,        /* server queue full; retry later */
+    MBE_ENOTREADY = -8     /* poll: request not finished yet */
+} mbe_status;
+
+/* The four algorithm variants of the reference (src/main.cu:121-122). */
+typedef enum {
+    MBE_CUMBE = 0, /* reverse scans + early-stop selection + work stealing */
+    MBE_NORS = 1,  /* forward scans over N(v) for the Q check and expansion */
+    MBE_NOES = 2,  /* candidate selection scans all of P, no early stop */
+    MBE_NOWS = 3   /* root pool only, no stealing below the root */
+} mbe_variant;
+
+typedef struct mbe_graph mbe_graph;   /* host CSR, sorted, deduplicated */
+
+/* Launch tuning. Zero fields take the library default (or the autotuner's
+ * cached choice when one is loaded). */
+typedef struct {
+    uint32_t workers;        /* persistent wave64 workers; 0 = auto */
+    uint32_t waves_per_cu;   /* occupancy target used when workers == 0 */
+    uint32_t lds_words;      /* per-worker LDS stack budget in u64 words */
+    uint32_t arena_words;    /* per-worker HBM arena in u64 words */
+    uint32_t list_cutoff;    /* P/Q size at or below which nodes use row
+                                (sorted-list style) scans instead of
+                                column bitmaps; 0 = default */
+    uint32_t donate_depth;   /* deepest level a worker donates from */
+    int32_t side;            /* -1 reference side choice, 0 enumerate U, 1 V */
+    uint32_t flags;          /* reserved, 0 */
+} mbe_tuning;
+
+typedef struct {
+    uint64_t count;        /* maximal bicliques */
+    uint64_t hash;         /* order-independent 64-bit hash (see DESIGN) */
+    double kernel_ms;      /* HIP-event time of the enumeration kernels */
+    double total_ms;       /* host wall time incl. preprocessing and copies */
+    uint64_t nodes;        /* enumeration-tree nodes visited */
+    uint64_t donated;      /* subtree tasks donated through the deque */
+    uint32_t launches;     /* kernel launches used */
+    uint32_t workers;      /* workers used */
+} mbe_result;
+
+const char* mbe_status_str(mbe_status s);
+int mbe_last_hip_error(void);       /* hipError_t of the last MBE_EHIP */
+const char* mbe_last_error(void);   /* human-readable detail, thread-local */
+
+/* ------------------------------------------------------------- graphs -- */
+mbe_status mbe_graph_load_bi(const char* path, mbe_graph** out);
+mbe_status mbe_graph_write_bi(const mbe_graph* g, const char* path);
+mbe_status mbe_graph_from_edge_list(const char* path, int skip_words, int skip_per_edge,
+                                    mbe_graph** out);
+/* Copies the CSR over U (rows): off_u[n_u + 1], adj_u[off_u[n_u]] with V ids.
+ * Sorts and deduplicates; rejects out-of-range ids with MBE_EINVAL. */
+mbe_status mbe_graph_from_csr(uint32_t n_u, uint32_t n_v, const uint32_t* off_u,
+                              const uint32_t* adj_u, mbe_graph** out);
+mbe_status mbe_graph_info(const mbe_graph* g, uint32_t* n_u, uint32_t* n_v, uint64_t* edges);
+/* Borrow the canonical CSR (valid until mbe_graph_free). */
+mbe_status mbe_graph_csr(const mbe_graph* g, const uint32_t** off_u, const uint32_t** adj_u);
+void mbe_graph_free(mbe_graph* g);
+
+/* Seeded synthetic generators (SplitMix64-seeded xoshiro256**). */
+mbe_status mbe_gen_chung_lu(uint32_t n_u, uint32_t n_v, uint64_t m, double gamma_u,
+                            double gamma_v, uint32_t max_deg, uint64_t seed, mbe_graph** out);
+mbe_status mbe_gen_er_planted(uint32_t n_u, uint32_t n_v, double p, uint32_t n_planted,
+                              uint32_t lo, uint32_t hi, double top_frac, uint64_t seed,
+                              mbe_graph** out);
+/* One graph of the batched workload's size distribution for this seed. */
+mbe_status mbe_gen_batch_graph(uint64_t seed, mbe_graph** out);
+
+/* -------------------------------------------------------------- solve -- */
+/* One graph, one stream, synchronous. tuning may be NULL. */
+mbe_status mbe_solve(const mbe_graph* g, mbe_variant variant, const mbe_tuning* tuning,
+                     mbe_result* out);
+
+/* Many graphs in one persistent launch sharing one task pool; per_graph[i]
+ * receives graph i's count and hash (timing fields are the batch's). */
+mbe_status mbe_batch_solve(const mbe_graph* const* graphs, size_t n, mbe_variant variant,
+                           const mbe_tuning* tuning, mbe_result* per_graph,
+                           mbe_result* total);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MBE_H */

@@ -1,0 +1,139 @@
+"""Pins the CPU oracle (oracle/mbe_oracle.c) before anything is compared to it.
+
+The reference ships no golden vectors or tests (it has no test directory), so
+the oracle is pinned by what the mathematics fixes:
+  * brute-force closure enumeration on tiny graphs (independent code path);
+  * closed forms: K_{a,b} -> 1, star -> 1, n disjoint edges -> n,
+    crown graph K_{n,n} minus a matching -> 2^n - 2, half graph -> n, empty -> 0,
+    disjoint unions add counts and hashes;
+  * an independent library routine: networkx maximal cliques of the graph with
+    both sides completed into cliques (its cliques meeting both sides are
+    exactly the maximal bicliques);
+  * invariances: enumerated-side choice, transpose (count), the four variant
+    switches (rs/es), root-parallel threads.
+"""
+import itertools
+
+import networkx as nx
+import numpy as np
+import pytest
+
+import graphs as G
+import oracle_lib as O
+
+MASK = (1 << 64) - 1
+
+
+def mix64(z):
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK
+    return z ^ (z >> 31)
+
+
+def spec_hash(a, b):
+    """h(A, B) written from the hash definition (file ids, U = rows)."""
+    su = sum(mix64(x ^ 0x9E3779B97F4A7C15) for x in a) & MASK
+    sv = sum(mix64(x ^ 0xC2B2AE3D27D4EB4F) for x in b) & MASK
+    rot = ((sv << 29) | (sv >> 35)) & MASK
+    return mix64(su ^ rot ^ ((len(a) * 0x165667B19E3779F9 + len(b) * 0xD6E8FEB86659FD93) & MASK))
+
+
+def nx_bicliques(n_u, n_v, off, adj):
+    g = nx.Graph()
+    us = [("u", i) for i in range(n_u)]
+    vs = [("v", j) for j in range(n_v)]
+    g.add_nodes_from(us + vs)
+    g.add_edges_from(itertools.combinations(us, 2))
+    g.add_edges_from(itertools.combinations(vs, 2))
+    for u in range(n_u):
+        for v in adj[off[u]:off[u + 1]]:
+            g.add_edge(("u", u), ("v", int(v)))
+    out = set()
+    for c in nx.find_cliques(g):
+        a = frozenset(i for s, i in c if s == "u")
+        b = frozenset(j for s, j in c if s == "v")
+        if a and b:
+            out.add((a, b))
+    return out
+
+
+SUITE = G.small_suite()
+
+
+@pytest.mark.parametrize("name,g", SUITE, ids=[n for n, _ in SUITE])
+def test_brute_force_agrees(name, g):
+    n_u, n_v, off, adj = g
+    if min(n_u, n_v) > 20:
+        pytest.skip("brute force limited to 20 vertices on the smaller side")
+    b = O.brute(n_u, n_v, off, adj)
+    for rs, es in [(1, 1), (0, 1), (1, 0), (0, 0)]:
+        r = O.solve(n_u, n_v, off, adj, rs=rs, es=es)
+        assert (r.count, r.hash) == (b.count, b.hash), (name, rs, es)
+
+
+@pytest.mark.parametrize("name,g", SUITE, ids=[n for n, _ in SUITE])
+def test_networkx_sets_and_spec_hash(name, g):
+    n_u, n_v, off, adj = g
+    sets, r = O.enumerate_sets(n_u, n_v, off, adj)
+    assert len(sets) == r.count == len(set(sets)), "each biclique reported once"
+    assert set(sets) == nx_bicliques(n_u, n_v, off, adj)
+    assert sum(spec_hash(a, b) for a, b in sets) & MASK == r.hash
+    for a, b in sets[:50]:
+        assert O.biclique_hash(a, b) == spec_hash(a, b)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 8, 12])
+def test_closed_forms(n):
+    assert O.solve(*G.complete(n, n + 2)).count == 1
+    assert O.solve(*G.star(n)).count == 1
+    assert O.solve(*G.matching(n)).count == n
+    assert O.solve(*G.half(n)).count == n
+    if n >= 2:
+        assert O.solve(*G.crown(n)).count == 2 ** n - 2
+    assert O.solve(*G.csr_from_edges(n, n, [])).count == 0
+
+
+def test_disjoint_union_adds():
+    g1, g2 = G.random_graph(20, 15, 0.3, 7), G.random_graph(12, 18, 0.4, 8)
+    r1, r2 = O.solve(*g1), O.solve(*g2)
+    # the second graph's ids shift in the union, so compare counts; hashes of
+    # a union with unshifted ids are pinned by the hash-sum test below
+    ru = O.solve(*G.disjoint_union(g1, g2))
+    assert ru.count == r1.count + r2.count
+
+
+def test_hash_sum_over_partition():
+    """The reported hash is the wrapping sum of per-biclique hashes."""
+    g = G.random_graph(25, 20, 0.35, 11)
+    sets, r = O.enumerate_sets(*g)
+    parts = [0, 0, 0]
+    for i, (a, b) in enumerate(sets):
+        parts[i % 3] = (parts[i % 3] + spec_hash(a, b)) & MASK
+    assert sum(parts) & MASK == r.hash
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_side_and_threads_invariance(seed):
+    g = G.random_graph(35 + seed, 28, 0.25, 200 + seed)
+    base = O.solve(*g)
+    for side in (0, 1):
+        r = O.solve(*g, side=side)
+        assert (r.count, r.hash) == (base.count, base.hash)
+    r = O.solve(*g, threads=4)
+    assert (r.count, r.hash) == (base.count, base.hash)
+    assert O.solve(*G.transpose(g)).count == base.count
+
+
+def test_rejects_bad_ids():
+    with pytest.raises(ValueError):
+        O.solve(2, 2, np.array([0, 1, 2], np.uint32), np.array([0, 5], np.uint32))
+
+
+def test_duplicates_are_removed():
+    n_u, n_v = 3, 3
+    off = np.array([0, 3, 4, 6], np.uint32)
+    adj = np.array([1, 0, 1, 2, 2, 2], np.uint32)
+    dedup = G.csr_from_edges(3, 3, [(0, 0), (0, 1), (1, 2), (2, 2)])
+    r1 = O.solve(n_u, n_v, off, adj)
+    r2 = O.solve(*dedup)
+    assert (r1.count, r1.hash) == (r2.count, r2.hash)
