@@ -44,6 +44,8 @@ class Tuning(C.Structure):
         ("arena_words", C.c_uint32),
         ("list_cutoff", C.c_uint32),
         ("donate_depth", C.c_uint32),
+        ("compact_ratio", C.c_uint32),
+        ("donate_interval", C.c_uint32),
         ("side", C.c_int32),
         ("flags", C.c_uint32),
     ]
@@ -59,10 +61,18 @@ class Result(C.Structure):
         ("donated", C.c_uint64),
         ("launches", C.c_uint32),
         ("workers", C.c_uint32),
+        ("sections", C.c_uint64 * 11),
     ]
 
     def as_dict(self) -> dict:
-        return {f: getattr(self, f) for f, _ in self._fields_}
+        d = {f: getattr(self, f) for f, _ in self._fields_}
+        d["sections"] = list(self.sections)
+        return d
+
+
+SECTION_NAMES = ["other", "root_build", "select", "lprime", "check", "expand", "report",
+                 "compact", "donate", "fetch", "wait"]
+FLAG_SECTIONS = 1
 
 
 _lib = None

@@ -10,6 +10,7 @@
 #include <mutex>
 #include <string>
 #include <vector>
+#include <unistd.h>
 
 #include "../../include/mbe.h"
 #include "engine.h"
@@ -17,6 +18,7 @@
 
 namespace mbe {
 hipError_t launch_mbe(const KParams& kp, uint32_t workers, hipStream_t stream);
+hipError_t launch_dq_init(unsigned long long* dq, uint32_t n_slots, hipStream_t stream);
 hipError_t set_lds_limit(uint32_t lds_words);
 }  // namespace mbe
 
@@ -72,10 +74,13 @@ struct Engine {
     std::mutex mu;
     bool init = false;
     int cus = 0;
+    size_t lds_per_cu = 160 * 1024;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    DevBuf arena, mark, cpos, ctl, graphs, data, ovf;
+    DevBuf arena, mark, cpos, ctl, graphs, data, ovf, slots, dq;
     uint32_t lds_set = 0;
+    unsigned long long* progress = nullptr;  // host-mapped, 2 per worker
+    size_t progress_slots = 0;
 
     mbe_status ensure_init() {
         if (init) return MBE_OK;
@@ -86,6 +91,7 @@ struct Engine {
         hipDeviceProp_t prop;
         HIPC(hipGetDeviceProperties(&prop, 0));
         cus = prop.multiProcessorCount;
+        if (prop.maxSharedMemoryPerMultiProcessor) lds_per_cu = prop.maxSharedMemoryPerMultiProcessor;
         HIPC(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         HIPC(hipEventCreate(&ev0));
         HIPC(hipEventCreate(&ev1));
@@ -100,20 +106,28 @@ Engine& engine() {
 }
 
 struct Tune {
-    uint32_t workers, lds_words, list_cutoff, donate_depth;
+    uint32_t workers, lds_words, list_cutoff, donate_depth, compact_ratio, donate_interval;
     uint64_t arena_words;
     int side;
+    uint32_t flags;
 };
 
-Tune resolve(const mbe_tuning* t, int cus) {
+Tune resolve(const mbe_tuning* t, int cus, size_t lds_per_cu) {
     Tune r;
-    uint32_t wpc = (t && t->waves_per_cu) ? t->waves_per_cu : 8;
+    r.lds_words = (t && t->lds_words) ? t->lds_words : 4096;
+    r.lds_words = std::max<uint32_t>(r.lds_words, mbe::kLdsHeaderWords + 64);
+    r.lds_words = std::min<uint32_t>(r.lds_words, (uint32_t)(lds_per_cu / 8));
+    // default occupancy: as many single-wave workers per CU as the LDS holds
+    uint32_t fit = (uint32_t)std::max<size_t>(1, lds_per_cu / ((size_t)r.lds_words * 8));
+    uint32_t wpc = (t && t->waves_per_cu) ? t->waves_per_cu : std::min<uint32_t>(fit, 16);
     r.workers = (t && t->workers) ? t->workers : (uint32_t)cus * wpc;
-    r.lds_words = (t && t->lds_words) ? t->lds_words : 2048;
     r.arena_words = (t && t->arena_words) ? t->arena_words : (1u << 18);
     r.list_cutoff = (t && t->list_cutoff) ? t->list_cutoff : 0;
     r.donate_depth = (t && t->donate_depth) ? t->donate_depth : 0;
+    r.compact_ratio = (t && t->compact_ratio) ? (t->compact_ratio == UINT32_MAX ? 0 : t->compact_ratio) : 2;
     r.side = t ? t->side : -1;
+    r.flags = t ? t->flags : 0;
+    r.donate_interval = (t && t->donate_interval) ? t->donate_interval : 64;
     return r;
 }
 
@@ -165,6 +179,30 @@ mbe_status upload(Engine& E, const std::vector<mbe::Prepared>& ps, std::vector<m
     return MBE_OK;
 }
 
+// Wait for the launch; with MBE_PROGRESS=<seconds> set, print the roots and
+// nodes the workers have finished (read from host-mapped slots) to stderr.
+hipError_t wait_with_progress(Engine& E, uint32_t nw, uint64_t n_tasks, int pass) {
+    const char* env = getenv("MBE_PROGRESS");
+    const double every = env ? atof(env) * 1000.0 : 0.0;
+    const double t0 = now_ms();
+    double last = t0;
+    for (;;) {
+        hipError_t q = hipEventQuery(E.ev1);
+        if (q != hipErrorNotReady) return q == hipSuccess ? hipEventSynchronize(E.ev1) : q;
+        usleep(2000);
+        if (every > 0 && now_ms() - last >= every) {
+            last = now_ms();
+            unsigned long long roots = 0, nodes = 0;
+            for (uint32_t w = 0; w < nw; w++) {
+                roots += ((volatile unsigned long long*)E.progress)[2 * w];
+                nodes += ((volatile unsigned long long*)E.progress)[2 * w + 1];
+            }
+            fprintf(stderr, "[mbe] pass %d: %.1f s, %llu/%llu roots, %llu nodes\n", pass,
+                    (last - t0) / 1000.0, roots, (unsigned long long)n_tasks, nodes);
+        }
+    }
+}
+
 // Run every root task of the uploaded graphs; roots whose local problem does
 // not fit a worker arena are retried with fewer workers and larger arenas.
 mbe_status run_tasks(Engine& E, const std::vector<mbe::Prepared>& ps,
@@ -176,13 +214,14 @@ mbe_status run_tasks(Engine& E, const std::vector<mbe::Prepared>& ps,
     const uint64_t cpos_len = max_nr + 1;
     const size_t ng = dg.size();
     // control block: cursor, overflow count, stats, results
-    const size_t ctl_words = 2 + mbe::kStatCount + 2 * ng;
+    const size_t ctl_words = 2 + mbe::kStatCount + mbe::kSecCount + 2 * ng;
     HIPC(E.ctl.ensure(ctl_words * 8));
     HIPC(hipMemsetAsync(E.ctl.p, 0, ctl_words * 8, E.stream));
     unsigned long long* cursor = E.ctl.as<unsigned long long>();
     unsigned long long* ovf_count = cursor + 1;
     unsigned long long* stats = cursor + 2;
-    unsigned long long* results = stats + mbe::kStatCount;
+    unsigned long long* secs = stats + mbe::kStatCount;
+    unsigned long long* results = secs + mbe::kSecCount;
 
     if (E.lds_set < tu.lds_words) {
         HIPC(mbe::set_lds_limit(tu.lds_words));
@@ -211,6 +250,11 @@ mbe_status run_tasks(Engine& E, const std::vector<mbe::Prepared>& ps,
                                 hipMemcpyHostToDevice, E.stream));
         }
         HIPC(hipMemsetAsync(cursor, 0, 16, E.stream));
+        const uint32_t n_slots = 1024;
+        const uint64_t slot_words = std::min<uint64_t>(std::max<uint64_t>(4096, arena_words / 8), 1u << 16);
+        HIPC(E.slots.ensure((size_t)n_slots * slot_words * 8));
+        HIPC(E.dq.ensure((4 + (size_t)n_slots) * 8));
+        HIPC(mbe::launch_dq_init(E.dq.as<unsigned long long>(), n_slots, E.stream));
         mbe::KParams kp{};
         kp.graphs = E.graphs.as<mbe::DevGraph>();
         kp.n_graphs = (uint32_t)ng;
@@ -228,15 +272,33 @@ mbe_status run_tasks(Engine& E, const std::vector<mbe::Prepared>& ps,
         kp.cpos_len = cpos_len;
         kp.lds_words = tu.lds_words;
         kp.list_cutoff = tu.list_cutoff;
+        kp.compact_ratio = tu.compact_ratio;
+        kp.sections = (tu.flags & MBE_FLAG_SECTIONS) ? secs : nullptr;
         kp.overflow_count = ovf_count;
         kp.overflow_list = ovf_list;
         kp.overflow_cap = ovf_cap;
         kp.donate_depth = tu.donate_depth;
-        uint32_t nw = (uint32_t)std::min<uint64_t>(workers, std::max<uint64_t>(n_tasks, 1));
+        kp.donate_interval = tu.donate_interval;
+        kp.slots = E.slots.as<uint64_t>();
+        kp.slot_words = slot_words;
+        kp.n_slots = n_slots;
+        kp.dq = E.dq.as<unsigned long long>();
+        // all workers start: those without a root task steal donated subtrees
+        uint32_t nw = workers;
+        if (E.progress_slots < 2 * (size_t)nw) {
+            if (E.progress) HIPC(hipHostFree(E.progress));
+            E.progress = nullptr;
+            HIPC(hipHostMalloc((void**)&E.progress, 2 * (size_t)nw * 8, hipHostMallocMapped | hipHostMallocCoherent));
+            E.progress_slots = 2 * (size_t)nw;
+        }
+        memset(E.progress, 0, 2 * (size_t)nw * 8);
+        unsigned long long* dprog = nullptr;
+        HIPC(hipHostGetDevicePointer((void**)&dprog, E.progress, 0));
+        kp.progress = dprog;
         HIPC(hipEventRecord(E.ev0, E.stream));
         HIPC(mbe::launch_mbe(kp, nw, E.stream));
         HIPC(hipEventRecord(E.ev1, E.stream));
-        HIPC(hipEventSynchronize(E.ev1));
+        HIPC(wait_with_progress(E, nw, n_tasks, pass));
         float ms = 0;
         HIPC(hipEventElapsedTime(&ms, E.ev0, E.ev1));
         kernel_ms += ms;
@@ -252,10 +314,13 @@ mbe_status run_tasks(Engine& E, const std::vector<mbe::Prepared>& ps,
         workers = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(workers, budget_words / arena_words));
         HIPC(hipMemsetAsync(ovf_count, 0, 8, E.stream));
     }
-    std::vector<unsigned long long> st(mbe::kStatCount), rr(2 * ng);
+    std::vector<unsigned long long> st(mbe::kStatCount), rr(2 * ng), sc(mbe::kSecCount);
     HIPC(hipMemcpy(st.data(), stats, st.size() * 8, hipMemcpyDeviceToHost));
+    HIPC(hipMemcpy(sc.data(), secs, sc.size() * 8, hipMemcpyDeviceToHost));
+    for (int i = 0; i < mbe::kSecCount; i++) out.sections[i] = sc[i];
     HIPC(hipMemcpy(rr.data(), results, rr.size() * 8, hipMemcpyDeviceToHost));
     res.assign(rr.begin(), rr.end());
+    if (st[mbe::kStatDropped]) return fail(MBE_ENOMEM, "donated subtree did not fit a worker arena");
     out.kernel_ms = kernel_ms;
     out.nodes = st[mbe::kStatNodes];
     out.donated = st[mbe::kStatDonated];
@@ -376,7 +441,7 @@ mbe_status mbe_batch_solve(const mbe_graph* const* graphs, size_t n, mbe_variant
     std::lock_guard<std::mutex> lock(E.mu);
     mbe_status s = E.ensure_init();
     if (s != MBE_OK) return s;
-    Tune tu = resolve(tuning, E.cus);
+    Tune tu = resolve(tuning, E.cus, E.lds_per_cu);
     if (tu.side < -1 || tu.side > 1) return fail(MBE_EINVAL, "bad side");
     std::vector<mbe::Prepared> ps(n);
     for (size_t i = 0; i < n; i++) mbe::prepare(graphs[i]->g, tu.side, ps[i]);

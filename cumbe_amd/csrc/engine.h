@@ -29,8 +29,29 @@ enum KStat : int {
     kStatStolen = 3,     // subtree tasks taken from the deque
     kStatOverflow = 4,   // root tasks deferred for a larger arena
     kStatBuildWords = 5, // adjacency entries read while building local problems
+    kStatDropped = 6,    // slots discarded (snapshot did not fit the thief's arena)
     kStatCount = 8
 };
+
+// Kernel time sections (DESECTION equivalent).
+enum KSec : int {
+    kSecOther = 0,    // loop control, frame bookkeeping
+    kSecBuild = 1,    // root localisation (distance-two scan, bit matrices)
+    kSecSelect = 2,   // candidate selection (ES)
+    kSecLPrime = 3,   // L' construction
+    kSecCheck = 4,    // maximality check of non-maximal nodes
+    kSecExpand = 5,   // maximality check + expansion of maximal nodes
+    kSecReport = 6,   // biclique hash and count
+    kSecCompact = 7,  // child re-localisation
+    kSecDonate = 8,   // work donation (snapshot + push)
+    kSecFetch = 9,    // stolen subtree load
+    kSecWait = 10,    // idle: deque empty, waiting for work or termination
+    kSecCount = 11
+};
+
+// Fixed per-worker LDS header (ES minima, nest stack, section counters) in
+// u64 words; a worker's LDS budget is at least this plus one frame.
+constexpr uint32_t kLdsHeaderWords = 344;
 
 // Deque slot header for donated subtrees (see mbe_kernel.hip).
 constexpr uint32_t kSlotHeaderWords = 8;
@@ -52,7 +73,8 @@ struct KParams {
     uint32_t* cpos;             // workers * cpos_len (R rank -> local index)
     uint64_t cpos_len;
     uint32_t lds_words;         // dynamic LDS per worker (u64 words)
-    uint32_t list_cutoff;       // P/Q popcount at or below which rows are scanned
+    uint32_t list_cutoff;       // |P|+|Q| at or below which a node tests rows (list form)
+    uint32_t compact_ratio;     // re-localise a child whose universe shrank this much (0 = never)
     // roots that did not fit the arena
     unsigned long long* overflow_count;
     uint64_t* overflow_list;
@@ -62,7 +84,11 @@ struct KParams {
     uint64_t slot_words;
     uint32_t n_slots;
     uint32_t donate_depth;
+    uint32_t donate_interval;   // nodes between a worker's donation checks
     unsigned long long* dq;     // [0] head, [1] tail, [2] idle, [3] pending, [4..] seq per slot
+    // host-mapped progress: 2 slots per worker (roots done, nodes), plain stores
+    unsigned long long* progress;
+    unsigned long long* sections;   // kSecCount tick sums, or null
 };
 
 }  // namespace mbe

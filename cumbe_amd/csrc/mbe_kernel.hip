@@ -3,9 +3,9 @@
 // Execution model: a persistent grid of single-wave workgroups; each wave64
 // is one worker (the reference's "block", src/mbe_noWS.cuh:101-615). A
 // worker claims root tasks from a global cursor (the reference's P_ptr,
-// src/mbe_noWS.cuh:123-133), localises the root's subproblem into bitmaps and
-// walks its set-enumeration subtree with an explicit stack of frames that live
-// in LDS and spill to the worker's HBM arena.
+// src/mbe_noWS.cuh:123-133), localises the root's subproblem into bit
+// matrices and walks its set-enumeration subtree with an explicit stack of
+// frames. Idle workers take donated subtrees from a deque in HBM.
 //
 // Local subproblem of root x (rank i): the universe of L is N(x) (d bits,
 // W = ceil(d/64) words); the candidates are the R vertices at distance two
@@ -14,19 +14,24 @@
 // Q (ranks > i) -- the root-level sets of src/mbe_noWS.cuh:135-154.
 //   rows[k]  = N(cand k)  n N(x)   as a d-bit set      (forward view)
 //   cols[j]  = N(u_j) n cands      as an nc-bit set    (reverse view)
-// A frame at level k holds L_k (W words), P_k and Q_k (Wc words each) plus
-// x, the running R hash sum and |R|.
+// A frame at level k holds x, the running R hash sum, |R|, |P| and |Q|, then
+// L_k (W words), P_k and Q_k (Wc words each). When everything fits, the whole
+// problem (hash terms, both bit matrices and all frames) lives in LDS;
+// otherwise the frames take the LDS and the rest sits in the worker's HBM
+// arena. All accesses go through generic pointers.
 //
 // Per node (src/mbe_cuMBE.cuh:345-553):
-//   candidate selection  x = argmin_{v in P} |row(v) & L|, reverse scan with
-//                        early stop at the previous minimum (ES);
-//   L' = row(x) & L;
+//   candidate selection  x = argmin_{v in P} |row(v) & L|, scanned in
+//                        descending candidate order with early stop at the
+//                        level's previous minimum (ES);
+//   L' = row(x) & L, and the list of its set bits;
 //   maximality + expansion, reverse scan (RS): A = AND_{j in L'} cols[j],
 //                        O = OR_{j in L'} cols[j]; non-maximal iff A & Q;
 //                        absorbed = A & P; P' = (O & P) \ A; Q' = O & Q;
-//   noRS: the same sets from forward row tests over P and Q.
+//   forward form (noRS, and small P u Q): the same sets from row tests.
 // Q' drops candidates with no neighbour in L' (they can never cover a
-// descendant's L), which removes work but no tree node.
+// descendant's L); a descent whose universe shrank by compact_ratio is
+// re-localised into a nested, compacted problem. Neither removes a tree node.
 #ifndef MBE_EMULATE
 #include <hip/hip_runtime.h>
 #endif
@@ -41,42 +46,119 @@ using u64 = unsigned long long;
 namespace {
 
 constexpr uint32_t kInf = 1u << 30;              // src/header.cuh:8
-constexpr uint32_t kPreMinLevels = 4096;         // per-level ES memory
-constexpr uint32_t kPreMinWords = kPreMinLevels / 2;
-constexpr uint32_t kFrameHdr = 4;                // x, SR, |R|, spare
+constexpr uint32_t kFrameHdr = 4;                // x, SR, |R|, |P| | |Q| << 32
+// Fixed per-worker LDS header: ES minima per level, the nest stack of
+// suspended problems, section counters. Problems use LDS after it.
+constexpr uint32_t kPreMinLevels = 256;          // deeper levels share the last
+constexpr uint32_t kNestMax = 48;                // nested compacted problems
+constexpr uint32_t kLdsPreMin = 0;                            // u32[256]
+constexpr uint32_t kLdsNest = kLdsPreMin + kPreMinLevels / 2;  // u64[4 * kNestMax]
+constexpr uint32_t kLdsSec = kLdsNest + 4 * kNestMax;          // u64[kSecCount]
+constexpr uint32_t kLdsHdr = (kLdsSec + kSecCount + 7) & ~7u;
+static_assert(kLdsHdr <= kLdsHeaderWords, "LDS header larger than the host reserves");
+constexpr uint32_t kArenaHdr = 0;                // arena words before problems
+constexpr uint32_t kLdsFramesMin = 24;           // frames an LDS tier must hold
+constexpr uint32_t kColVecMax = 8;               // Wc up to which columns are
+                                                 // scanned lane-per-L'-bit
 
 enum Variant : int { kCuMBE = 0, kNoRS = 1, kNoES = 2, kNoWS = 3 };
 
 __device__ __forceinline__ uint32_t lane() { return threadIdx.x; }
-
-__device__ __forceinline__ u64 wsum64(u64 v) {
-    for (int o = 32; o; o >>= 1) v += __shfl_xor(v, o);
-    return v;
-}
-__device__ __forceinline__ uint32_t wsum32(uint32_t v) {
-    for (int o = 32; o; o >>= 1) v += __shfl_xor(v, o);
-    return v;
-}
-__device__ __forceinline__ uint32_t wmin32(uint32_t v) {
-    for (int o = 32; o; o >>= 1) v = min(v, (uint32_t)__shfl_xor(v, o));
-    return v;
-}
-// exclusive prefix sum across the wave
-__device__ __forceinline__ uint32_t wscan_excl(uint32_t v) {
-    uint32_t x = v;
-    for (int o = 1; o < 64; o <<= 1) {
-        uint32_t y = __shfl_up(x, o);
-        if ((int)lane() >= o) x += y;
-    }
-    return x - v;
-}
-// value known to be equal in all lanes -> scalar register
+__device__ __forceinline__ uint32_t uni32(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 __device__ __forceinline__ u64 uni64(u64 v) {
     uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
     uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
     return ((u64)hi << 32) | lo;
 }
-__device__ __forceinline__ uint32_t uni32(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint32_t rdlane32(uint32_t v, uint32_t l) {
+    return __builtin_amdgcn_readlane(v, l);
+}
+__device__ __forceinline__ u64 rdlane64(u64 v, uint32_t l) {
+    uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, l);
+    uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), l);
+    return ((u64)hi << 32) | lo;
+}
+__device__ __forceinline__ u64 lanemask_lt() { return (1ull << lane()) - 1; }
+
+// Wave reductions with DPP: row_shr 1,2,4,8 builds an inclusive scan inside
+// each 16-lane row, row_bcast:15 and row_bcast:31 carry it across rows, and
+// lane 63 holds the total.
+#define MBE_DPP(old, v, ctrl, rm) \
+    ((uint32_t)__builtin_amdgcn_update_dpp((int)(old), (int)(v), ctrl, rm, 0xf, false))
+#define MBE_DPP64(old, v, ctrl, rm)                                                 \
+    ((((u64)MBE_DPP((uint32_t)((old) >> 32), (uint32_t)((v) >> 32), ctrl, rm)) << 32) | \
+     (u64)MBE_DPP((uint32_t)(old), (uint32_t)(v), ctrl, rm))
+
+__device__ __forceinline__ uint32_t scan_add32(uint32_t v) {  // inclusive
+    v += MBE_DPP(0, v, 0x111, 0xf);
+    v += MBE_DPP(0, v, 0x112, 0xf);
+    v += MBE_DPP(0, v, 0x114, 0xf);
+    v += MBE_DPP(0, v, 0x118, 0xf);
+    v += MBE_DPP(0, v, 0x142, 0xa);
+    v += MBE_DPP(0, v, 0x143, 0xc);
+    return v;
+}
+__device__ __forceinline__ uint32_t red_add32(uint32_t v) { return rdlane32(scan_add32(v), 63); }
+__device__ __forceinline__ uint32_t red_min32(uint32_t v) {
+    v = min(v, MBE_DPP(0xffffffffu, v, 0x111, 0xf));
+    v = min(v, MBE_DPP(0xffffffffu, v, 0x112, 0xf));
+    v = min(v, MBE_DPP(0xffffffffu, v, 0x114, 0xf));
+    v = min(v, MBE_DPP(0xffffffffu, v, 0x118, 0xf));
+    v = min(v, MBE_DPP(0xffffffffu, v, 0x142, 0xa));
+    v = min(v, MBE_DPP(0xffffffffu, v, 0x143, 0xc));
+    return rdlane32(v, 63);
+}
+__device__ __forceinline__ u64 red_add64(u64 v) {
+    v += MBE_DPP64(0ull, v, 0x111, 0xf);
+    v += MBE_DPP64(0ull, v, 0x112, 0xf);
+    v += MBE_DPP64(0ull, v, 0x114, 0xf);
+    v += MBE_DPP64(0ull, v, 0x118, 0xf);
+    v += MBE_DPP64(0ull, v, 0x142, 0xa);
+    v += MBE_DPP64(0ull, v, 0x143, 0xc);
+    return rdlane64(v, 63);
+}
+__device__ __forceinline__ u64 red_and64(u64 v) {
+    v &= MBE_DPP64(~0ull, v, 0x111, 0xf);
+    v &= MBE_DPP64(~0ull, v, 0x112, 0xf);
+    v &= MBE_DPP64(~0ull, v, 0x114, 0xf);
+    v &= MBE_DPP64(~0ull, v, 0x118, 0xf);
+    v &= MBE_DPP64(~0ull, v, 0x142, 0xa);
+    v &= MBE_DPP64(~0ull, v, 0x143, 0xc);
+    return rdlane64(v, 63);
+}
+__device__ __forceinline__ u64 red_or64(u64 v) {
+    v |= MBE_DPP64(0ull, v, 0x111, 0xf);
+    v |= MBE_DPP64(0ull, v, 0x112, 0xf);
+    v |= MBE_DPP64(0ull, v, 0x114, 0xf);
+    v |= MBE_DPP64(0ull, v, 0x118, 0xf);
+    v |= MBE_DPP64(0ull, v, 0x142, 0xa);
+    v |= MBE_DPP64(0ull, v, 0x143, 0xc);
+    return rdlane64(v, 63);
+}
+
+__device__ __forceinline__ void wsync() { __syncthreads(); }
+
+// Per-section time accounting (the reference's DESECTION clocks,
+// src/header.cuh:25-32), enabled when KParams::sections is set. Ticks are
+// wall_clock64() units (100 MHz) summed over workers.
+struct Sec {
+    u64* t;  // kSecCount counters in the worker's LDS header
+    u64 last;
+    bool on;
+    __device__ __forceinline__ void init(bool en, u64* lds) {
+        on = en;
+        t = lds + kLdsSec;
+        if (lane() < kSecCount) t[lane()] = 0;
+        last = en ? (u64)wall_clock64() : 0;
+    }
+    __device__ __forceinline__ void mark(int i) {
+        if (on) {
+            u64 now = (u64)wall_clock64();
+            if (lane() == 0) t[i] += now - last;
+            last = now;
+        }
+    }
+};
 
 __device__ __forceinline__ uint32_t find_graph(const KParams& kp, uint64_t t) {
     uint32_t lo = 0, hi = kp.n_graphs;
@@ -97,33 +179,111 @@ __device__ __forceinline__ uint32_t find_j(const uint32_t* pre, uint32_t d, uint
     return lo;
 }
 
+// One local subproblem: a root's, a stolen snapshot's or a compacted child's.
 struct Local {
     uint32_t x, d, W, nc, Wc, kx, fs, nlds, maxdepth;
+    uint32_t lds_off, lds_end;   // LDS words [lds_off, lds_end) in use
+    bool in_lds;                 // whole problem in LDS
+    size_t base, end;            // arena words [base, end) in use
     const uint32_t* Nx;
     uint32_t* pre;
-    u64* hl;
     uint32_t* cand;
-    u64* hc;
+    u64* hl;                     // hash term per L bit
+    u64* hc;                     // hash term per candidate
     u64* rows;
     u64* cols;
-    uint32_t* list;
-    u64* hstack;
-    u64* lds;
+    uint32_t* list;              // scratch: candidate compaction
+    uint32_t* lbits;             // scratch: set bits of the newest L'
+    u64* hstack;                 // frames beyond the LDS ones
+    u64* fbase;                  // LDS frames
 
     __device__ __forceinline__ u64* frame(uint32_t k) const {
         uint32_t i = k - 1;
-        return i < nlds ? lds + (size_t)i * fs : hstack + (size_t)(i - nlds) * fs;
+        return i < nlds ? fbase + (size_t)i * fs : hstack + (size_t)(i - nlds) * fs;
     }
 };
 
-// sum of hash terms over the set bits of L (W words), all lanes get it
-__device__ __forceinline__ u64 sum_l(const Local& S, const u64* L) {
-    u64 s = 0;
-    for (uint32_t lw = 0; lw < S.W; lw++) {
-        u64 word = uni64(L[lw]);
-        if ((word >> lane()) & 1) s += S.hl[lw * 64 + lane()];
+__device__ __forceinline__ uint32_t pcount(const u64* F) { return (uint32_t)F[3]; }
+__device__ __forceinline__ u64 pq(uint32_t p, uint32_t q) { return (u64)p | ((u64)q << 32); }
+
+// Place a problem of size (d, nc) from LDS word lds_off / arena word off:
+// everything in LDS when it fits; else the scratch lists and the frames in
+// LDS and the data (hash terms, bit matrices) in the arena; else only as many
+// frames as fit in LDS.
+__device__ bool layout(const KParams& kp, Local& S, u64* arena, u64* lds, size_t off,
+                       uint32_t lds_off) {
+    S.W = (S.d + 63) >> 6;
+    S.Wc = (S.nc + 63) >> 6;
+    S.maxdepth = min(S.d, S.nc) + 2;
+    S.fs = kFrameHdr + S.W + 2 * S.Wc;
+    S.lds_off = lds_off;
+    S.base = off;
+    const size_t data = (size_t)S.d + S.nc + (size_t)S.nc * S.W + (size_t)S.d * S.Wc;
+    const size_t scratch = (S.nc + 1) / 2 + 1 + (S.d + 1) / 2 + 1;
+    // LDS frames wanted: the real depth is far below min(d, nc) + 2, so
+    // only the first kLdsFramesMin frames count when choosing a tier
+    const size_t frames = (size_t)min(S.maxdepth, kLdsFramesMin) * S.fs;
+    const size_t room = lds_off < kp.lds_words ? kp.lds_words - lds_off : 0;
+    S.in_lds = data + scratch + frames <= room;
+    const bool scratch_lds = S.in_lds || scratch + frames <= room;
+    u64* lp = lds + lds_off;  // LDS cursor
+    u64* ap = arena + off;    // arena cursor
+    u64* p = S.in_lds ? lp : ap;
+    S.hl = p;
+    p += S.d;
+    S.hc = p;
+    p += S.nc;
+    S.rows = p;
+    p += (size_t)S.nc * S.W;
+    S.cols = p;
+    p += (size_t)S.d * S.Wc;
+    if (S.in_lds) lp = p; else ap = p;
+    p = scratch_lds ? lp : ap;
+    S.list = (uint32_t*)p;
+    p += (S.nc + 1) / 2 + 1;
+    S.lbits = (uint32_t*)p;
+    p += (S.d + 1) / 2 + 1;
+    if (scratch_lds) lp = p; else ap = p;
+    const size_t lroom = room - (size_t)(lp - (lds + lds_off));
+    S.nlds = (uint32_t)min((size_t)S.maxdepth, lroom / S.fs);
+    S.fbase = lp;
+    lp += (size_t)S.nlds * S.fs;
+    S.hstack = ap;
+    ap += (size_t)(S.maxdepth - S.nlds) * S.fs;
+    S.lds_end = (uint32_t)(lp - lds);
+    S.end = (size_t)(ap - arena);
+    return S.end <= kp.arena_words;
+}
+
+// cols = transpose(rows) with 64 ballots per 64x64 bit block.
+__device__ void build_cols(const Local& S) {
+    for (uint32_t b = 0; b < S.Wc; b++) {
+        const uint32_t k = b * 64 + lane();
+        const bool valid = k < S.nc;
+        for (uint32_t w = 0; w < S.W; w++) {
+            u64 rw = valid ? S.rows[(size_t)k * S.W + w] : 0;
+            u64 mine = 0;
+            if (__ballot(rw != 0)) {
+                for (uint32_t t = 0; t < 64; t++) {
+                    u64 bal = __ballot((rw >> t) & 1);
+                    if (lane() == t) mine = bal;
+                }
+            }
+            const uint32_t j = w * 64 + lane();
+            if (j < S.d) S.cols[(size_t)j * S.Wc + b] = mine;
+        }
     }
-    return wsum64(s);
+    wsync();
+}
+
+// Sum of hash terms over the first nl entries of the L' bit list.
+__device__ __forceinline__ u64 sum_l(const Local& S, uint32_t nl) {
+    u64 s = 0;
+    for (uint32_t c0 = 0; c0 < nl; c0 += 64) {
+        uint32_t l = c0 + lane();
+        if (l < nl) s += S.hl[S.lbits[l]];
+    }
+    return red_add64(s);
 }
 
 __device__ __forceinline__ void record(const DevGraph& G, u64 sl, u64 sr, u64 nl, u64 nr, u64& cnt,
@@ -134,361 +294,802 @@ __device__ __forceinline__ void record(const DevGraph& G, u64 sl, u64 sr, u64 nl
 }
 
 // Candidate selection at a level >= 1 (src/mbe_cuMBE.cuh:345-390): scan P in
-// descending candidate order, keep the first strict minimum of |row(v) & L|;
-// with ES stop as soon as the running minimum equals the level's previous
-// minimum. Removes the chosen candidate from P and returns it.
+// descending candidate order (words high to low, one lane per set bit) and
+// keep the first strict minimum of |row(v) & L|; with ES stop as soon as the
+// running minimum equals the level's previous minimum
+// (src/mbe_cuMBE.cuh:352,365-366; noES scans all, src/mbe_noES.cuh:352).
+// Removes the chosen candidate from P and returns it.
 template <int V>
 __device__ uint32_t select_x(const Local& S, u64* F, uint32_t* premin_lvl) {
     u64* L = F + kFrameHdr;
     u64* P = L + S.W;
-    // compact P's set bits into S.list in ascending order
-    uint32_t n = 0;
-    for (uint32_t w0 = 0; w0 < S.Wc; w0 += 64) {
-        uint32_t w = w0 + lane();
-        u64 pw = w < S.Wc ? P[w] : 0;
-        uint32_t c = __popcll(pw);
-        uint32_t idx = n + wscan_excl(c);
-        while (pw) {
-            uint32_t b = __builtin_ctzll(pw);
-            pw &= pw - 1;
-            S.list[idx++] = w * 64 + b;
-        }
-        n += wsum32(c);
-    }
-    __syncthreads();
     const uint32_t pm = *premin_lvl;
     uint32_t best = kInf, arg = 0;
     bool nostop = (V == kNoES);
-    bool done = false;
-    for (int top = (int)n - 1; top >= 0 && !done; top -= 64) {
-        int i = top - (int)lane();
-        bool valid = i >= 0;
-        uint32_t v = valid ? S.list[i] : 0;
+    for (int w = (int)S.Wc - 1; w >= 0; w--) {
+        const u64 pw = uni64(P[w]);
+        if (!pw) continue;
+        const bool valid = (pw >> lane()) & 1;
         uint32_t c = kInf;
         if (valid) {
             c = 0;
-            const u64* r = S.rows + (size_t)v * S.W;
-            for (uint32_t w = 0; w < S.W; w++) c += __popcll(r[w] & L[w]);
+            const u64* r = S.rows + (size_t)(w * 64 + lane()) * S.W;
+            for (uint32_t i = 0; i < S.W; i++) c += __popcll(r[i] & L[i]);
         }
         if (!nostop) {
-            u64 m = __ballot(valid && c <= pm);
+            const u64 m = __ballot(valid && c <= pm);
             if (m) {
-                int s = __builtin_ctzll(m);
-                uint32_t cs = __shfl(c, s);
+                const uint32_t s = 63 - __builtin_clzll(m);
+                const uint32_t cs = rdlane32(c, s);
                 if (cs == pm) {
                     best = cs;
-                    arg = __shfl(v, s);
-                    done = true;
+                    arg = w * 64 + s;
                     break;
                 }
                 nostop = true;
             }
         }
-        uint32_t cm = wmin32(c);
+        const uint32_t cm = red_min32(c);
         if (cm < best) {
-            int l = __builtin_ctzll(__ballot(c == cm));
             best = cm;
-            arg = __shfl(v, l);
+            arg = w * 64 + (63 - __builtin_clzll(__ballot(c == cm)));
         }
     }
-    arg = uni32(arg);
     if (lane() == 0) {
         *premin_lvl = best;
         P[arg >> 6] &= ~(1ull << (arg & 63));
+        F[3] -= 1;  // |P|
     }
-    __syncthreads();
+    wsync();
     return arg;
 }
 
 struct NodeOut {
     bool maximal;
     bool descend;
-    uint32_t nl;    // |L'|
     uint32_t ab;    // absorbed count
     u64 abh;        // absorbed hash sum
+    uint32_t pn;    // |P'|
+    uint32_t qn;    // |Q'|
+    uint32_t un;    // |P' u Q'|
 };
 
-// Build L' = row(x) & L into the child frame; return |L'|.
-__device__ __forceinline__ uint32_t make_lprime(const Local& S, const u64* L, uint32_t x, u64* Lc) {
-    uint32_t c = 0;
+// L' = row(x) & L into the child frame and its set-bit list; returns |L'|.
+__device__ uint32_t make_lprime(const Local& S, const u64* L, uint32_t x, u64* Lc) {
     const u64* r = S.rows + (size_t)x * S.W;
-    for (uint32_t w = lane(); w < S.W; w += 64) {
-        u64 lw = r[w] & L[w];
-        Lc[w] = lw;
-        c += __popcll(lw);
+    uint32_t nl = 0;
+    for (uint32_t w0 = 0; w0 < S.W; w0 += 64) {
+        const uint32_t w = w0 + lane();
+        u64 lw = 0;
+        if (w < S.W) {
+            lw = r[w] & L[w];
+            Lc[w] = lw;
+        }
+        const uint32_t nw = min(64u, S.W - w0);
+        for (uint32_t i = 0; i < nw; i++) {
+            const u64 word = rdlane64(lw, i);
+            if ((word >> lane()) & 1)
+                S.lbits[nl + __popcll(word & lanemask_lt())] = (w0 + i) * 64 + lane();
+            nl += __popcll(word);
+        }
     }
-    c = wsum32(c);
-    __syncthreads();
-    return c;
+    wsync();
+    return nl;
 }
 
-// Reverse-scan maximality check and expansion (src/mbe_noWS.cuh:215-289).
+// Reverse-scan maximality check and expansion (src/mbe_noWS.cuh:215-289):
+// AND/OR of the L' columns. Small Wc: one lane per L' bit, one word at a
+// time; large Wc: one lane per column word, walking the L' bits.
 __device__ NodeOut process_rs(const Local& S, u64* F, u64* Fc, uint32_t nl) {
     const u64* P = F + kFrameHdr + S.W;
     const u64* Q = P + S.Wc;
-    const u64* Lc = Fc + kFrameHdr;
     u64* Pc = Fc + kFrameHdr + S.W;
     u64* Qc = Pc + S.Wc;
-    NodeOut o{true, false, nl, 0, 0};
-    bool nonmax = false, anyp = false;
-    uint32_t ab = 0;
+    NodeOut o{true, false, 0, 0, 0, 0, 0};
     u64 abh = 0;
-    for (uint32_t w0 = 0; w0 < S.Wc; w0 += 64) {
-        uint32_t w = w0 + lane();
-        bool valid = w < S.Wc;
-        u64 a = ~0ull, ob = 0;
-        for (uint32_t lw = 0; lw < S.W; lw++) {
-            u64 word = uni64(Lc[lw]);
-            while (word) {
-                uint32_t j = lw * 64 + __builtin_ctzll(word);
-                word &= word - 1;
+    if (S.Wc <= kColVecMax) {
+        for (uint32_t w = 0; w < S.Wc; w++) {
+            u64 a = ~0ull, ob = 0;
+            for (uint32_t c0 = 0; c0 < nl; c0 += 64) {
+                const uint32_t l = c0 + lane();
+                u64 ca = ~0ull, co = 0;
+                if (l < nl) {
+                    const u64 c = S.cols[(size_t)S.lbits[l] * S.Wc + w];
+                    ca = c;
+                    co = c;
+                }
+                a &= red_and64(ca);
+                ob |= red_or64(co);
+            }
+            const u64 q = uni64(Q[w]), p = uni64(P[w]);
+            if (a & q) {
+                o.maximal = false;
+                break;
+            }
+            const u64 absd = a & p, pn = ob & p & ~a, qn = ob & q;
+            if (lane() == 0) {
+                Pc[w] = pn;
+                Qc[w] = qn;
+            }
+            o.ab += __popcll(absd);
+            o.pn += __popcll(pn);
+            o.qn += __popcll(qn);
+            o.un += __popcll(pn | qn);
+            if ((absd >> lane()) & 1) abh += S.hc[w * 64 + lane()];
+        }
+    } else {
+        bool nonmax = false;
+        uint32_t ab = 0, pnc = 0, qnc = 0, unc = 0;
+        for (uint32_t w0 = 0; w0 < S.Wc; w0 += 64) {
+            const uint32_t w = w0 + lane();
+            const bool valid = w < S.Wc;
+            u64 a = ~0ull, ob = 0;
+            uint32_t l = 0;
+            for (; l + 4 <= nl; l += 4) {
+                const uint32_t j0 = S.lbits[l], j1 = S.lbits[l + 1];
+                const uint32_t j2 = S.lbits[l + 2], j3 = S.lbits[l + 3];
                 if (valid) {
-                    u64 c = S.cols[(size_t)j * S.Wc + w];
+                    const u64 c0 = S.cols[(size_t)j0 * S.Wc + w], c1 = S.cols[(size_t)j1 * S.Wc + w];
+                    const u64 c2 = S.cols[(size_t)j2 * S.Wc + w], c3 = S.cols[(size_t)j3 * S.Wc + w];
+                    a &= c0 & c1 & c2 & c3;
+                    ob |= c0 | c1 | c2 | c3;
+                }
+            }
+            for (; l < nl; l++) {
+                const uint32_t j = S.lbits[l];
+                if (valid) {
+                    const u64 c = S.cols[(size_t)j * S.Wc + w];
                     a &= c;
                     ob |= c;
                 }
             }
+            const u64 q = valid ? Q[w] : 0, p = valid ? P[w] : 0;
+            nonmax |= (a & q) != 0;
+            const u64 absd = a & p, pn = ob & p & ~a, qn = ob & q;
+            if (valid) {
+                Pc[w] = pn;
+                Qc[w] = qn;
+            }
+            ab += __popcll(absd);
+            pnc += __popcll(pn);
+            qnc += __popcll(qn);
+            unc += __popcll(pn | qn);
+            u64 t = absd;
+            while (t) {
+                const uint32_t b = __builtin_ctzll(t);
+                t &= t - 1;
+                abh += S.hc[w * 64 + b];
+            }
         }
-        u64 q = valid ? Q[w] : 0, p = valid ? P[w] : 0;
-        nonmax |= (a & q) != 0;
-        u64 absd = a & p;
-        u64 pn = ob & p & ~a;
-        u64 qn = ob & q;
-        if (valid) {
-            Pc[w] = pn;
-            Qc[w] = qn;
-        }
-        anyp |= pn != 0;
-        ab += __popcll(absd);
-        while (absd) {
-            uint32_t b = __builtin_ctzll(absd);
-            absd &= absd - 1;
-            abh += S.hc[w * 64 + b];
-        }
+        o.maximal = __ballot(nonmax) == 0;
+        o.ab = red_add32(ab);
+        o.pn = red_add32(pnc);
+        o.qn = red_add32(qnc);
+        o.un = red_add32(unc);
     }
-    o.maximal = __ballot(nonmax) == 0;
-    o.descend = o.maximal && __ballot(anyp) != 0;
-    o.ab = wsum32(ab);
-    o.abh = wsum64(abh);
-    __syncthreads();
+    o.abh = red_add64(abh);
+    o.descend = o.maximal && o.pn != 0;
+    wsync();
     return o;
 }
 
-// Forward-scan variant (noRS, src/mbe_noRS.cuh:216-287): every Q and P
-// candidate's row is tested against L' directly.
+// Forward form (noRS, src/mbe_noRS.cuh:216-287, and nodes whose P u Q is
+// small): each candidate's row is tested against L', one lane per set bit.
 __device__ NodeOut process_fwd(const Local& S, u64* F, u64* Fc, uint32_t nl) {
     const u64* P = F + kFrameHdr + S.W;
     const u64* Q = P + S.Wc;
     const u64* Lc = Fc + kFrameHdr;
     u64* Pc = Fc + kFrameHdr + S.W;
     u64* Qc = Pc + S.Wc;
-    NodeOut o{true, false, nl, 0, 0};
-    bool nonmax = false;
-    for (uint32_t w0 = 0; w0 < S.Wc; w0 += 64) {
-        uint32_t w = w0 + lane();
-        u64 qw = w < S.Wc ? Q[w] : 0, qn = 0;
-        while (qw) {
-            uint32_t b = __builtin_ctzll(qw);
-            qw &= qw - 1;
-            const u64* r = S.rows + (size_t)(w * 64 + b) * S.W;
-            bool cov = true, touch = false;
-            for (uint32_t lw = 0; lw < S.W; lw++) {
-                u64 l = Lc[lw], x = r[lw] & l;
-                cov &= x == l;
-                touch |= x != 0;
+    NodeOut o{true, false, 0, 0, 0, 0, 0};
+    for (uint32_t w = 0; w < S.Wc; w++) {
+        const u64 qw = uni64(Q[w]);
+        u64 qn = 0;
+        if (qw) {
+            const bool valid = (qw >> lane()) & 1;
+            bool cov = false, touch = false;
+            if (valid) {
+                const u64* r = S.rows + (size_t)(w * 64 + lane()) * S.W;
+                cov = true;
+                for (uint32_t i = 0; i < S.W; i++) {
+                    const u64 l = Lc[i], x = r[i] & l;
+                    cov &= x == l;
+                    touch |= x != 0;
+                }
             }
-            nonmax |= cov;
-            if (touch) qn |= 1ull << b;
+            if (__ballot(cov)) {
+                o.maximal = false;
+                break;
+            }
+            qn = __ballot(touch);
         }
-        if (w < S.Wc) Qc[w] = qn;
+        if (lane() == 0) Qc[w] = qn;
+        o.qn += __popcll(qn);
     }
-    o.maximal = __ballot(nonmax) == 0;
     if (!o.maximal) {
-        __syncthreads();
+        wsync();
         return o;
     }
-    bool anyp = false;
-    uint32_t ab = 0;
+    wsync();
     u64 abh = 0;
-    for (uint32_t w0 = 0; w0 < S.Wc; w0 += 64) {
-        uint32_t w = w0 + lane();
-        u64 pw = w < S.Wc ? P[w] : 0, pn = 0;
-        while (pw) {
-            uint32_t b = __builtin_ctzll(pw);
-            pw &= pw - 1;
-            uint32_t k = w * 64 + b;
-            const u64* r = S.rows + (size_t)k * S.W;
+    for (uint32_t w = 0; w < S.Wc; w++) {
+        const u64 pw = uni64(P[w]);
+        u64 pn = 0, absd = 0;
+        if (pw) {
+            const bool valid = (pw >> lane()) & 1;
             uint32_t c = 0;
-            for (uint32_t lw = 0; lw < S.W; lw++) c += __popcll(r[lw] & Lc[lw]);
-            if (c == nl) {
-                ab++;
-                abh += S.hc[k];
-            } else if (c) {
-                pn |= 1ull << b;
+            if (valid) {
+                const u64* r = S.rows + (size_t)(w * 64 + lane()) * S.W;
+                for (uint32_t i = 0; i < S.W; i++) c += __popcll(r[i] & Lc[i]);
             }
+            absd = __ballot(valid && c == nl);
+            pn = __ballot(valid && c != 0 && c != nl);
+            if ((absd >> lane()) & 1) abh += S.hc[w * 64 + lane()];
         }
-        if (w < S.Wc) Pc[w] = pn;
-        anyp |= pn != 0;
+        if (lane() == 0) Pc[w] = pn;
+        o.ab += __popcll(absd);
+        o.pn += __popcll(pn);
+        o.un += __popcll(pn | uni64(Qc[w]));
     }
-    o.descend = __ballot(anyp) != 0;
-    o.ab = wsum32(ab);
-    o.abh = wsum64(abh);
-    __syncthreads();
+    o.abh = red_add64(abh);
+    o.descend = o.pn != 0;
+    wsync();
     return o;
 }
 
-// Localise root x and enumerate its subtree. Returns false when the local
-// problem does not fit the arena (the root is then deferred).
+// ------------------------------------------------------------------ deque
+// Donated subtrees travel through a bounded multi-producer multi-consumer
+// ring of fixed-size slots in HBM (per-slot sequence numbers, Vyukov's
+// bounded queue). dq[0] head, dq[1] tail, dq[2] idle workers, dq[3] pending
+// tasks (queued or running), dq[4 + i] sequence number of slot i.
+// Slot layout (u64 words): [0] graph, [1] d, [2] nc, [3] R hash sum, [4] |R|,
+// then hl[d], hc[nc], rows[nc * W], P[Wc], Q[Wc]: a LEVEL(L, R, P, Q) call
+// over a compacted universe (L = all d bits).
+
+__device__ __forceinline__ u64 ld_acq(unsigned long long* p) {
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ u64 ld_rlx(unsigned long long* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_rel(unsigned long long* p, u64 v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Claim a slot to write (lane 0 decides); returns its position or -1 if full.
+__device__ int64_t dq_claim_push(const KParams& kp) {
+    int64_t r = -1;
+    if (lane() == 0) {
+        unsigned long long* tail = kp.dq + 1;
+        u64 pos = ld_rlx(tail);
+        for (int it = 0; it < 64; it++) {
+            u64 seq = ld_acq(kp.dq + 4 + (pos % kp.n_slots));
+            int64_t dif = (int64_t)(seq - pos);
+            if (dif == 0) {
+                u64 old = atomicCAS(tail, pos, pos + 1);
+                if (old == pos) { r = (int64_t)pos; break; }
+                pos = old;
+            } else if (dif < 0) {
+                break;  // full
+            } else {
+                pos = ld_rlx(tail);
+            }
+        }
+    }
+    return (int64_t)uni64((u64)r);
+}
+
+__device__ void dq_publish(const KParams& kp, u64 pos) {
+    __threadfence();
+    wsync();
+    if (lane() == 0) st_rel(kp.dq + 4 + (pos % kp.n_slots), pos + 1);
+}
+
+// Claim a filled slot to read; returns its position or -1 if empty.
+__device__ int64_t dq_claim_pop(const KParams& kp) {
+    int64_t r = -1;
+    if (lane() == 0) {
+        unsigned long long* head = kp.dq;
+        u64 pos = ld_rlx(head);
+        for (int it = 0; it < 64; it++) {
+            u64 seq = ld_acq(kp.dq + 4 + (pos % kp.n_slots));
+            int64_t dif = (int64_t)(seq - (pos + 1));
+            if (dif == 0) {
+                u64 old = atomicCAS(head, pos, pos + 1);
+                if (old == pos) { r = (int64_t)pos; break; }
+                pos = old;
+            } else if (dif < 0) {
+                break;  // empty
+            } else {
+                pos = ld_rlx(head);
+            }
+        }
+    }
+    r = (int64_t)uni64((u64)r);
+    __threadfence();
+    return r;
+}
+
+__device__ void dq_release(const KParams& kp, u64 pos) {
+    __threadfence();
+    wsync();
+    if (lane() == 0) st_rel(kp.dq + 4 + (pos % kp.n_slots), pos + kp.n_slots);
+}
+
+// Count |L| and |P u Q (u x)| of a frame, for sizing a compaction.
+__device__ __forceinline__ void frame_sizes(const Local& S, const u64* L, const u64* P, const u64* Q,
+                                            uint32_t xq, uint32_t& dn, uint32_t& n) {
+    uint32_t a = 0, b = 0;
+    for (uint32_t w = lane(); w < S.W; w += 64) a += __popcll(L[w]);
+    for (uint32_t w = lane(); w < S.Wc; w += 64) {
+        u64 cw = P[w] | Q[w];
+        if (w == (xq >> 6)) cw |= 1ull << (xq & 63);
+        b += __popcll(cw);
+    }
+    dn = red_add32(a);
+    n = red_add32(b);
+}
+
+// Compact (L, P, Q [+ x into Q]) of problem S into the arrays of a smaller
+// universe: L's set bits renumbered 0..dn-1, the candidates of P u Q (u x)
+// renumbered 0..n-1 in order. Writes hl', hc', rows' (dn-bit rows), P', Q'.
+// S.list and S.lbits are overwritten (scratch).
+__device__ void compact_sets(const Local& S, const u64* L, const u64* P, const u64* Q, uint32_t xq,
+                             u64* hl2, u64* hc2, u64* rows2, u64* P2, u64* Q2, uint32_t dn,
+                             uint32_t n) {
+    const uint32_t Wn = (dn + 63) >> 6;
+    // candidate list (ascending) and L bit list
+    uint32_t n0 = 0;
+    for (uint32_t w = 0; w < S.Wc; w++) {
+        u64 cw = uni64(P[w] | Q[w]);
+        if (w == (xq >> 6)) cw |= 1ull << (xq & 63);
+        if ((cw >> lane()) & 1) S.list[n0 + __popcll(cw & lanemask_lt())] = w * 64 + lane();
+        n0 += __popcll(cw);
+    }
+    uint32_t d0 = 0;
+    for (uint32_t w = 0; w < S.W; w++) {
+        const u64 word = uni64(L[w]);
+        if ((word >> lane()) & 1) {
+            const uint32_t at = d0 + __popcll(word & lanemask_lt());
+            S.lbits[at] = w * 64 + lane();
+            hl2[at] = S.hl[w * 64 + lane()];
+        }
+        d0 += __popcll(word);
+    }
+    wsync();
+    // candidates: hash terms, rows bit-extracted over L, P'/Q' by ballot
+    for (uint32_t c0 = 0; c0 < n; c0 += 64) {
+        const uint32_t kn = c0 + lane();
+        const bool valid = kn < n;
+        const uint32_t k = valid ? S.list[kn] : 0;
+        if (valid) hc2[kn] = S.hc[k];
+        const u64* r = S.rows + (size_t)k * S.W;
+        for (uint32_t w2 = 0; w2 < Wn; w2++) {
+            u64 acc = 0;
+            const uint32_t lim = min(64u, dn - w2 * 64);
+            for (uint32_t t = 0; t < lim; t++) {
+                const uint32_t j = S.lbits[w2 * 64 + t];
+                if (valid) acc |= ((r[j >> 6] >> (j & 63)) & 1ull) << t;
+            }
+            if (valid) rows2[(size_t)kn * Wn + w2] = acc;
+        }
+        const bool inP = valid && ((P[k >> 6] >> (k & 63)) & 1);
+        const bool inQ = valid && ((((Q[k >> 6] >> (k & 63)) & 1)) || k == xq);
+        const u64 bp = __ballot(inP), bq = __ballot(inQ);
+        if (lane() == 0) {
+            P2[c0 >> 6] = bp;
+            Q2[c0 >> 6] = bq;
+        }
+    }
+    wsync();
+}
+
+// Write the compaction of level k (L_k, remaining P_k, Q_k u {x_k}) into a
+// slot as a LEVEL call over its own universe.
+__device__ void write_snapshot(const KParams& kp, const Local& S, u64* F, uint32_t g, u64* slot) {
+    const u64* L = F + kFrameHdr;
+    const u64* P = L + S.W;
+    const u64* Q = P + S.Wc;
+    const uint32_t xk = (uint32_t)F[0];
+    uint32_t dn, n;
+    frame_sizes(S, L, P, Q, xk, dn, n);
+    const uint32_t Wn = (dn + 63) >> 6, Wcn = (n + 63) >> 6;
+    u64* hl2 = slot + 8;
+    u64* hc2 = hl2 + dn;
+    u64* rows2 = hc2 + n;
+    u64* P2 = rows2 + (size_t)n * Wn;
+    u64* Q2 = P2 + Wcn;
+    compact_sets(S, L, P, Q, xk, hl2, hc2, rows2, P2, Q2, dn, n);
+    if (lane() == 0) {
+        slot[0] = g;
+        slot[1] = dn;
+        slot[2] = n;
+        slot[3] = F[1];
+        slot[4] = F[2];
+    }
+}
+
+__device__ __forceinline__ void fill_full_l(const Local& S, u64* L1) {
+    for (uint32_t w = lane(); w < S.W; w += 64)
+        L1[w] = (w == S.W - 1 && (S.d & 63)) ? ((1ull << (S.d & 63)) - 1) : ~0ull;
+}
+
+// Load a slot into the worker's memory as a ready frame 1; false if too big.
+__device__ bool load_snapshot(const KParams& kp, Local& S, u64* arena, u64* lds, const u64* slot) {
+    S.d = (uint32_t)slot[1];
+    S.nc = (uint32_t)slot[2];
+    S.x = 0xFFFFFFFFu;
+    if (!layout(kp, S, arena, lds, kArenaHdr, kLdsHdr)) return false;
+    const u64* hl2 = slot + 8;
+    const u64* hc2 = hl2 + S.d;
+    const u64* rows2 = hc2 + S.nc;
+    const u64* P2 = rows2 + (size_t)S.nc * S.W;
+    const u64* Q2 = P2 + S.Wc;
+    for (uint32_t i = lane(); i < S.d; i += 64) S.hl[i] = hl2[i];
+    for (uint32_t i = lane(); i < S.nc; i += 64) S.hc[i] = hc2[i];
+    const size_t nr = (size_t)S.nc * S.W;
+    for (size_t i = lane(); i < nr; i += 64) S.rows[i] = rows2[i];
+    wsync();
+    build_cols(S);
+    u64* F1 = S.frame(1);
+    u64* L1 = F1 + kFrameHdr;
+    u64* P1 = L1 + S.W;
+    u64* Q1 = P1 + S.Wc;
+    fill_full_l(S, L1);
+    uint32_t pc = 0, qc = 0;
+    for (uint32_t w = lane(); w < S.Wc; w += 64) {
+        P1[w] = P2[w];
+        Q1[w] = Q2[w];
+        pc += __popcll(P2[w]);
+        qc += __popcll(Q2[w]);
+    }
+    pc = red_add32(pc);
+    qc = red_add32(qc);
+    if (lane() == 0) {
+        F1[1] = slot[3];
+        F1[2] = slot[4];
+        F1[3] = pq(pc, qc);
+    }
+    wsync();
+    return true;
+}
+
+// Replace the pending descent into child frame Fc of S by a nested problem
+// S2 over the child's own universe (L', P' u Q'), placed after S in the
+// arena and after S's live frames in LDS. Returns false if it does not fit.
+__device__ bool compact_child(const KParams& kp, const Local& S, uint32_t k, const u64* Fc,
+                              Local& S2, u64* arena, u64* lds, uint32_t dn, uint32_t n,
+                              uint32_t pn) {
+    S2.d = dn;
+    S2.nc = n;
+    S2.x = 0xFFFFFFFFu;
+    // after S's LDS data and scratch and its live frames (1..k+1)
+    const uint32_t lds_from =
+        (uint32_t)(S.fbase - lds) + min(k + 1, S.nlds) * S.fs;
+    if (!layout(kp, S2, arena, lds, S.end, lds_from)) return false;
+    const u64* L = Fc + kFrameHdr;
+    const u64* P = L + S.W;
+    const u64* Q = P + S.Wc;
+    u64* F1 = S2.frame(1);
+    u64* L1 = F1 + kFrameHdr;
+    u64* P1 = L1 + S2.W;
+    u64* Q1 = P1 + S2.Wc;
+    const u64 sr = Fc[1], nr = Fc[2];
+    compact_sets(S, L, P, Q, 0xFFFFFFFFu, S2.hl, S2.hc, S2.rows, P1, Q1, dn, n);
+    build_cols(S2);
+    fill_full_l(S2, L1);
+    if (lane() == 0) {
+        F1[1] = sr;
+        F1[2] = nr;
+        F1[3] = pq(pn, n - pn);
+    }
+    wsync();
+    return true;
+}
+
+// Size in words of the snapshot of frame F (x_k joins Q).
+__device__ size_t snapshot_words(const Local& S, const u64* F) {
+    const u64* L = F + kFrameHdr;
+    const u64* P = L + S.W;
+    const u64* Q = P + S.Wc;
+    uint32_t dn, n;
+    frame_sizes(S, L, P, Q, (uint32_t)F[0], dn, n);
+    const uint32_t Wn = (dn + 63) >> 6, Wcn = (n + 63) >> 6;
+    return dn ? 8 + (size_t)dn + n + (size_t)n * Wn + 2 * (size_t)Wcn : ~(size_t)0;
+}
+
+// Rebuild a suspended problem from its nest record.
+__device__ __forceinline__ void unpack(const KParams& kp, const u64* rec, Local& T, u64* arena,
+                                       u64* lds, uint32_t& k, uint32_t& depth0) {
+    T.d = (uint32_t)rec[1];
+    T.nc = (uint32_t)(rec[1] >> 32);
+    T.x = 0xFFFFFFFFu;
+    layout(kp, T, arena, lds, (size_t)rec[0], (uint32_t)rec[2]);
+    k = (uint32_t)(rec[2] >> 32);
+    depth0 = (uint32_t)rec[3];
+}
+
+// Donate the shallowest level that still has unselected candidates and
+// whose snapshot fits a slot: suspended outer problems first (levels up to
+// the one their nested child hangs from), then the current problem's levels
+// below `cur`. Returns true when a subtree was handed over.
+__device__ bool try_donate(const KParams& kp, const Local& S, uint32_t cur, uint32_t g,
+                           u64* arena, u64* lds, const u64* nest, uint32_t np) {
+    for (uint32_t i = 0; i <= np; i++) {
+        Local T;
+        uint32_t top, d0;
+        if (i < np) {
+            unpack(kp, nest + 4 * i, T, arena, lds, top, d0);
+        } else {
+            T = S;
+            top = cur - 1;
+        }
+        if (kp.donate_depth) top = min(top, kp.donate_depth);
+        for (uint32_t k = 1; k <= top; k++) {
+            u64* F = T.frame(k);
+            if (pcount(F) == 0) continue;
+            if (snapshot_words(T, F) > kp.slot_words) continue;
+            if (lane() == 0) atomicAdd(kp.dq + 3, 1ull);  // pending, before publishing
+            int64_t pos = dq_claim_push(kp);
+            if (pos < 0) {
+                if (lane() == 0) atomicAdd(kp.dq + 3, ~0ull);
+                return false;
+            }
+            u64* slot = (u64*)kp.slots + (size_t)((u64)pos % kp.n_slots) * kp.slot_words;
+            write_snapshot(kp, T, F, g, slot);
+            dq_publish(kp, (u64)pos);
+            u64* P = F + kFrameHdr + T.W;
+            for (uint32_t w = lane(); w < T.Wc; w += 64) P[w] = 0;
+            if (lane() == 0) F[3] &= ~0xFFFFFFFFull;  // |P| = 0
+            wsync();
+            return true;
+        }
+    }
+    return false;
+}
+
+// Iterative DFS over levels >= 1 from a ready frame 1 of S
+// (src/mbe_noWS.cuh:337-615), with donation to the deque in place of
+// src/mbe_cuMBE.cuh's level-1 publish/steal, and descents into children
+// whose universe shrank by compact_ratio re-localised as nested problems.
 template <int V>
-__device__ bool solve_root(const KParams& kp, const DevGraph& G, uint32_t x, u64* arena, u64* lds,
-                           u64* mark, uint32_t* cpos, uint32_t* premin, u64& cnt, u64& hs,
-                           u64& nodes, u64& build) {
-    Local S;
+__device__ void dfs(const KParams& kp, const DevGraph& G, uint32_t g, Local& S, u64* arena,
+                    u64* lds, uint32_t* premin, u64& cnt, u64& hs, u64& nodes, u64& donated,
+                    Sec& sec) {
+    u64* nest = lds + kLdsNest;
+    uint32_t np = 0, depth0 = 0, k = 1;
+    u64 next_check = nodes + kp.donate_interval;
+    for (;;) {
+        u64* F = S.frame(k);
+        const u64 cnts = uni64(F[3]);
+        const uint32_t pc = (uint32_t)cnts, qc = (uint32_t)(cnts >> 32);
+        if (pc == 0) {
+            k--;
+            if (k == 0) {
+                if (np == 0) break;
+                np--;
+                unpack(kp, nest + 4 * np, S, arena, lds, k, depth0);
+            }
+            // ascend: Q[k] <- Q[k] u {x[k]} (src/mbe_noWS.cuh:590-601)
+            u64* Fp = S.frame(k);
+            if (lane() == 0) {
+                const uint32_t xp = (uint32_t)Fp[0];
+                u64* Qp = Fp + kFrameHdr + S.W + S.Wc;
+                Qp[xp >> 6] |= 1ull << (xp & 63);
+                Fp[3] += 1ull << 32;  // |Q|
+            }
+            wsync();
+            continue;
+        }
+        // donate when workers idle and the deque holds fewer tasks than
+        // there are idle workers; at most once per donate_interval nodes
+        if (V != kNoWS && (k > 1 || np > 0) && nodes >= next_check) {
+            next_check = nodes + kp.donate_interval;
+            u64 want = 0;
+            if (lane() == 0) {
+                const u64 idle = ld_rlx(kp.dq + 2);
+                const u64 queued = ld_rlx(kp.dq + 1) - ld_rlx(kp.dq);
+                want = idle > queued;
+            }
+            if (uni64(want) && try_donate(kp, S, k, g, arena, lds, nest, np)) {
+                donated++;
+                sec.mark(kSecDonate);
+                continue;  // this level may have been handed over
+            }
+        }
+        sec.mark(kSecOther);
+        u64* L = F + kFrameHdr;
+        uint32_t* pm = premin + min(depth0 + k, kPreMinLevels - 1);
+        const uint32_t xk = select_x<V>(S, F, pm);
+        nodes++;
+        if (lane() == 0) F[0] = xk;
+        sec.mark(kSecSelect);
+        u64* Fc = S.frame(k + 1);
+        const uint32_t nl = make_lprime(S, L, xk, Fc + kFrameHdr);
+        sec.mark(kSecLPrime);
+        const NodeOut o = (V == kNoRS || pc - 1 + qc <= kp.list_cutoff)
+                              ? process_fwd(S, F, Fc, nl)
+                              : process_rs(S, F, Fc, nl);
+        sec.mark(o.maximal ? kSecExpand : kSecCheck);
+        if (o.maximal) {
+            const u64 sl = sum_l(S, nl);
+            const u64 sr = F[1] + S.hc[xk] + o.abh;
+            const u64 nr = F[2] + 1 + o.ab;
+            record(G, sl, sr, nl, nr, cnt, hs);
+            sec.mark(kSecReport);
+            if (o.descend) {
+                if (lane() == 0) {
+                    Fc[1] = sr;
+                    Fc[2] = nr;
+                    Fc[3] = pq(o.pn, o.qn);
+                }
+                wsync();
+                const uint32_t Wcn = (o.un + 63) >> 6, Wn = (nl + 63) >> 6;
+                if (kp.compact_ratio && np < kNestMax && o.pn >= 2 &&
+                    (Wcn * kp.compact_ratio <= S.Wc || Wn * kp.compact_ratio <= S.W)) {
+                    Local S2;
+                    const bool okc = compact_child(kp, S, k, Fc, S2, arena, lds, nl, o.un, o.pn);
+                    sec.mark(kSecCompact);
+                    if (okc) {
+                        if (lane() == 0) {
+                            u64* r = nest + 4 * np;
+                            r[0] = S.base;
+                            r[1] = (u64)S.d | ((u64)S.nc << 32);
+                            r[2] = (u64)S.lds_off | ((u64)k << 32);
+                            r[3] = depth0;
+                        }
+                        wsync();
+                        np++;
+                        depth0 += k;
+                        S = S2;
+                        k = 1;
+                        continue;
+                    }
+                }
+                k++;
+                continue;
+            }
+        }
+        if (lane() == 0) {
+            u64* Q = L + S.W + S.Wc;
+            Q[xk >> 6] |= 1ull << (xk & 63);
+            F[3] += 1ull << 32;  // |Q|
+        }
+        wsync();
+    }
+}
+
+// Localise root x (rows/cols over N(x) x distance-two candidates) and
+// process the root node. Returns 0 done, 1 DFS from frame 1 needed, -1 does
+// not fit the arena.
+template <int V>
+__device__ int build_root(const KParams& kp, const DevGraph& G, uint32_t x, u64* arena, u64* lds,
+                          u64* mark, uint32_t* cpos, Local& S, u64& cnt, u64& hs, u64& nodes,
+                          u64& build) {
     S.x = x;
     const uint32_t s0 = G.offR[x];
     S.d = G.offR[x + 1] - s0;
     S.Nx = G.adjR + s0;
-    S.W = (S.d + 63) >> 6;
-    S.lds = lds;
-    if (S.d == 0) return true;
-    size_t off = kPreMinWords;
+    if (S.d == 0) return 0;
+    size_t off = kArenaHdr;
     S.pre = (uint32_t*)(arena + off);
     off += (S.d + 2) / 2;
-    S.hl = arena + off;
-    off += S.d;
-    if (off > kp.arena_words) return false;
+    S.cand = (uint32_t*)(arena + off);
+    const size_t off_cand = off;
+    if (off + S.d > kp.arena_words) return -1;
 
-    // degree prefix over N(x) and the L hash terms
+    // degree prefix over N(x)
     uint32_t carry = 0;
     for (uint32_t j0 = 0; j0 < S.d; j0 += 64) {
-        uint32_t j = j0 + lane(), dg = 0;
+        const uint32_t j = j0 + lane();
+        uint32_t dg = 0;
         if (j < S.d) {
-            uint32_t u = S.Nx[j];
+            const uint32_t u = S.Nx[j];
             dg = G.offL[u + 1] - G.offL[u];
-            S.hl[j] = G.hL[u];
         }
-        uint32_t ex = wscan_excl(dg);
-        if (j < S.d) S.pre[j] = carry + ex;
-        carry += wsum32(dg);
+        const uint32_t inc = scan_add32(dg);
+        if (j < S.d) S.pre[j] = carry + inc - dg;
+        carry += rdlane32(inc, 63);
     }
     if (lane() == 0) S.pre[S.d] = carry;
     const uint32_t T = carry;
     build += T;
-    __syncthreads();
+    wsync();
 
     // discover the distance-two candidates (bitmap over R ranks)
     for (uint32_t f0 = 0; f0 < T; f0 += 64) {
-        uint32_t f = f0 + lane();
+        const uint32_t f = f0 + lane();
         if (f < T) {
-            uint32_t j = find_j(S.pre, S.d, f);
-            uint32_t u = S.Nx[j];
-            uint32_t v = G.adjL[G.offL[u] + f - S.pre[j]];
+            const uint32_t j = find_j(S.pre, S.d, f);
+            const uint32_t u = S.Nx[j];
+            const uint32_t v = G.adjL[G.offL[u] + f - S.pre[j]];
             if (v != x) atomicOr(&mark[v >> 6], 1ull << (v & 63));
         }
     }
     __threadfence();
-    __syncthreads();
+    wsync();
     const uint32_t nwords = (G.nR + 63) >> 6;
     uint32_t ncl = 0, kxl = 0;
     for (uint32_t w = lane(); w < nwords; w += 64) {
-        u64 m = mark[w];
+        const u64 m = mark[w];
         ncl += __popcll(m);
         if (w < (x >> 6)) kxl += __popcll(m);
         else if (w == (x >> 6)) kxl += __popcll(m & ((1ull << (x & 63)) - 1));
     }
-    S.nc = wsum32(ncl);
-    S.kx = wsum32(kxl);
-    S.Wc = (S.nc + 63) >> 6;
-    S.maxdepth = min(S.d, S.nc) + 2;
-    S.fs = kFrameHdr + S.W + 2 * S.Wc;
-    S.nlds = kp.lds_words / S.fs;
-    size_t o_cand = off;
-    off += (S.nc + 1) / 2;
-    size_t o_hc = off;
-    off += S.nc;
-    size_t o_rows = off;
-    off += (size_t)S.nc * S.W;
-    size_t o_cols = off;
-    off += (size_t)S.d * S.Wc;
-    size_t o_list = off;
-    off += (S.nc + 1) / 2 + 1;
-    size_t o_stack = off;
-    if (S.maxdepth > S.nlds) off += (size_t)(S.maxdepth - S.nlds) * S.fs;
-    if (off > kp.arena_words) {
+    S.nc = red_add32(ncl);
+    S.kx = red_add32(kxl);
+    off = off_cand + (S.nc + 1) / 2;
+    if (!layout(kp, S, arena, lds, off, kLdsHdr)) {
         for (uint32_t w = lane(); w < nwords; w += 64) mark[w] = 0;
         __threadfence();
-        __syncthreads();
-        return false;
+        wsync();
+        return -1;
     }
-    S.cand = (uint32_t*)(arena + o_cand);
-    S.hc = arena + o_hc;
-    S.rows = arena + o_rows;
-    S.cols = arena + o_cols;
-    S.list = (uint32_t*)(arena + o_list);
-    S.hstack = arena + o_stack;
+    for (uint32_t j = lane(); j < S.d; j += 64) S.hl[j] = G.hL[S.Nx[j]];
 
     // assign local indices in rank order; clear the bitmap
     uint32_t k0 = 0;
     for (uint32_t w0 = 0; w0 < nwords; w0 += 64) {
-        uint32_t w = w0 + lane();
+        const uint32_t w = w0 + lane();
         u64 m = w < nwords ? mark[w] : 0;
-        uint32_t c = __popcll(m);
-        uint32_t k = k0 + wscan_excl(c);
+        const uint32_t c = __popcll(m);
+        const uint32_t inc = scan_add32(c);
+        uint32_t k = k0 + inc - c;
         if (m) mark[w] = 0;
         while (m) {
-            uint32_t b = __builtin_ctzll(m);
+            const uint32_t b = __builtin_ctzll(m);
             m &= m - 1;
-            uint32_t v = w * 64 + b;
+            const uint32_t v = w * 64 + b;
             S.cand[k] = v;
             cpos[v] = k;
             S.hc[k] = G.hR[v];
             k++;
         }
-        k0 += wsum32(c);
+        k0 += rdlane32(inc, 63);
     }
     const size_t nz = (size_t)S.nc * S.W + (size_t)S.d * S.Wc;
     for (size_t i = lane(); i < nz; i += 64) S.rows[i] = 0;
     __threadfence();
-    __syncthreads();
+    wsync();
     // rows and cols of the local bit matrix
     for (uint32_t f0 = 0; f0 < T; f0 += 64) {
-        uint32_t f = f0 + lane();
+        const uint32_t f = f0 + lane();
         if (f < T) {
-            uint32_t j = find_j(S.pre, S.d, f);
-            uint32_t u = S.Nx[j];
-            uint32_t v = G.adjL[G.offL[u] + f - S.pre[j]];
+            const uint32_t j = find_j(S.pre, S.d, f);
+            const uint32_t u = S.Nx[j];
+            const uint32_t v = G.adjL[G.offL[u] + f - S.pre[j]];
             if (v != x) {
-                uint32_t k = cpos[v];
+                const uint32_t k = cpos[v];
                 atomicOr(&S.rows[(size_t)k * S.W + (j >> 6)], 1ull << (j & 63));
                 atomicOr(&S.cols[(size_t)j * S.Wc + (k >> 6)], 1ull << (k & 63));
             }
         }
     }
     __threadfence();
-    __syncthreads();
+    wsync();
 
     // ---- root node: L' = N(x), P = cands below kx, Q = the rest
     u64* F1 = S.frame(1);
     u64* L1 = F1 + kFrameHdr;
     u64* P1 = L1 + S.W;
     u64* Q1 = P1 + S.Wc;
-    for (uint32_t w = lane(); w < S.W; w += 64)
-        L1[w] = (w == S.W - 1 && (S.d & 63)) ? ((1ull << (S.d & 63)) - 1) : ~0ull;
+    fill_full_l(S, L1);
     bool nonmax = false;
     uint32_t ab = 0;
     u64 abh = 0;
     for (uint32_t c0 = 0; c0 < S.nc && !nonmax; c0 += 64) {
-        uint32_t k = c0 + lane();
-        bool valid = k < S.nc, full = false;
+        const uint32_t k = c0 + lane();
+        const bool valid = k < S.nc;
+        bool full = false;
         if (valid) {
-            uint32_t pc = 0;
+            uint32_t c = 0;
             const u64* r = S.rows + (size_t)k * S.W;
-            for (uint32_t w = 0; w < S.W; w++) pc += __popcll(r[w]);
-            full = pc == S.d;
+            for (uint32_t w = 0; w < S.W; w++) c += __popcll(r[w]);
+            full = c == S.d;
         }
-        bool isP = valid && k < S.kx, isQ = valid && k >= S.kx;
+        const bool isP = valid && k < S.kx, isQ = valid && k >= S.kx;
         if (__ballot(full && isQ)) nonmax = true;
-        u64 bp = __ballot(isP && !full), bq = __ballot(isQ);
+        const u64 bp = __ballot(isP && !full), bq = __ballot(isQ);
         if (lane() == 0) {
             P1[c0 >> 6] = bp;
             Q1[c0 >> 6] = bq;
@@ -499,70 +1100,22 @@ __device__ bool solve_root(const KParams& kp, const DevGraph& G, uint32_t x, u64
         }
     }
     nodes++;
-    if (nonmax) return true;
-    ab = wsum32(ab);
-    abh = wsum64(abh);
-    {
-        u64 sl = 0;
-        for (uint32_t j = lane(); j < S.d; j += 64) sl += S.hl[j];
-        sl = wsum64(sl);
-        u64 sr = G.hR[x] + abh, nr = 1 + ab;
-        record(G, sl, sr, S.d, nr, cnt, hs);
-        if (S.kx == ab) return true;   // P' empty
-        if (lane() == 0) {
-            F1[1] = sr;
-            F1[2] = nr;
-        }
+    if (nonmax) return 0;
+    ab = red_add32(ab);
+    abh = red_add64(abh);
+    u64 sl = 0;
+    for (uint32_t j = lane(); j < S.d; j += 64) sl += S.hl[j];
+    sl = red_add64(sl);
+    const u64 sr = G.hR[x] + abh, nr = 1 + ab;
+    record(G, sl, sr, S.d, nr, cnt, hs);
+    if (S.kx == ab) return 0;  // P' empty
+    if (lane() == 0) {
+        F1[1] = sr;
+        F1[2] = nr;
+        F1[3] = pq(S.kx - ab, S.nc - S.kx);
     }
-    __syncthreads();
-
-    // ---- iterative DFS over levels >= 1 (src/mbe_noWS.cuh:337-615)
-    uint32_t k = 1;
-    while (k >= 1) {
-        u64* F = S.frame(k);
-        u64* L = F + kFrameHdr;
-        u64* P = L + S.W;
-        u64* Q = P + S.Wc;
-        bool anyP = false;
-        for (uint32_t w = lane(); w < S.Wc; w += 64) anyP |= P[w] != 0;
-        if (__ballot(anyP) == 0) {
-            // level exhausted: ascend, Q[k-1] <- Q[k-1] u {x[k-1]}
-            k--;
-            if (k >= 1) {
-                u64* Fp = S.frame(k);
-                uint32_t xp = (uint32_t)Fp[0];
-                u64* Qp = Fp + kFrameHdr + S.W + S.Wc;
-                if (lane() == 0) Qp[xp >> 6] |= 1ull << (xp & 63);
-                __syncthreads();
-            }
-            continue;
-        }
-        uint32_t* pm = premin + min(k, kPreMinLevels - 1);
-        uint32_t xk = select_x<V>(S, F, pm);
-        nodes++;
-        if (lane() == 0) F[0] = xk;
-        u64* Fc = S.frame(k + 1);
-        uint32_t nl = make_lprime(S, L, xk, Fc + kFrameHdr);
-        NodeOut o = (V == kNoRS) ? process_fwd(S, F, Fc, nl) : process_rs(S, F, Fc, nl);
-        if (o.maximal) {
-            u64 sl = sum_l(S, Fc + kFrameHdr);
-            u64 sr = F[1] + S.hc[xk] + o.abh;
-            u64 nr = F[2] + 1 + o.ab;
-            record(G, sl, sr, nl, nr, cnt, hs);
-            if (o.descend) {
-                if (lane() == 0) {
-                    Fc[1] = sr;
-                    Fc[2] = nr;
-                }
-                __syncthreads();
-                k++;
-                continue;
-            }
-        }
-        if (lane() == 0) Q[xk >> 6] |= 1ull << (xk & 63);
-        __syncthreads();
-    }
-    return true;
+    wsync();
+    return 1;
 }
 
 template <int V>
@@ -572,41 +1125,134 @@ __global__ __launch_bounds__(64) void mbe_persistent(KParams kp) {
     u64* arena = (u64*)kp.arena + (size_t)wid * kp.arena_words;
     u64* mark = (u64*)kp.mark + (size_t)wid * kp.mark_words;
     uint32_t* cpos = kp.cpos + (size_t)wid * kp.cpos_len;
-    uint32_t* premin = (uint32_t*)arena;
+    uint32_t* premin = (uint32_t*)(lds_mem + kLdsPreMin);
     for (uint32_t i = lane(); i < kPreMinLevels; i += 64) premin[i] = 1;  // src/main.cu:86
-    u64 nodes = 0, roots = 0, build = 0, ovf = 0;
+    u64 nodes = 0, roots = 0, build = 0, ovf = 0, donated = 0, stolen = 0, dropped = 0;
+    bool roots_left = true, idle = false;
+    uint32_t backoff = 1;
+    unsigned long long* pending = kp.dq + 3;
+    Sec sec;
+    sec.init(kp.sections != nullptr, lds_mem);
     for (;;) {
-        __syncthreads();
-        u64 t = 0;
-        if (lane() == 0) t = atomicAdd(kp.cursor, 1ull);
-        t = uni64(__shfl(t, 0));
-        if (t >= kp.n_tasks) break;
-        const uint64_t task = kp.task_list ? kp.task_list[t] : t;
-        const uint32_t g = find_graph(kp, task);
-        const DevGraph G = kp.graphs[g];
-        const uint32_t x = (uint32_t)(task - G.task_base);
-        u64 cnt = 0, hs = 0;
-        bool ok = solve_root<V>(kp, G, x, arena, lds_mem, mark, cpos, premin, cnt, hs, nodes, build);
-        if (!ok) {
-            ovf++;
+        wsync();
+        if (roots_left) {
+            u64 t = 0;
             if (lane() == 0) {
-                u64 idx = atomicAdd(kp.overflow_count, 1ull);
-                if (idx < kp.overflow_cap) kp.overflow_list[idx] = task;
+                atomicAdd(pending, 1ull);
+                t = atomicAdd(kp.cursor, 1ull);
+                if (t >= kp.n_tasks) atomicAdd(pending, ~0ull);
             }
-        } else {
-            roots++;
-            if (lane() == 0 && cnt) {
-                atomicAdd(&kp.results[2 * g], cnt);
-                atomicAdd(&kp.results[2 * g + 1], hs);
+            t = uni64(t);
+            if (t < kp.n_tasks) {
+                if (idle && lane() == 0) atomicAdd(kp.dq + 2, ~0ull);
+                idle = false;
+                const uint64_t task = kp.task_list ? kp.task_list[t] : t;
+                const uint32_t g = find_graph(kp, task);
+                const DevGraph G = kp.graphs[g];
+                const uint32_t x = (uint32_t)(task - G.task_base);
+                u64 cnt = 0, hs = 0;
+                Local S;
+                sec.mark(kSecOther);
+                const int st = build_root<V>(kp, G, x, arena, lds_mem, mark, cpos, S, cnt, hs,
+                                             nodes, build);
+                sec.mark(kSecBuild);
+                if (st > 0) dfs<V>(kp, G, g, S, arena, lds_mem, premin, cnt, hs, nodes, donated, sec);
+                if (st < 0) {
+                    ovf++;
+                    if (lane() == 0) {
+                        u64 idx = atomicAdd(kp.overflow_count, 1ull);
+                        if (idx < kp.overflow_cap) kp.overflow_list[idx] = task;
+                    }
+                } else {
+                    roots++;
+                    if (lane() == 0 && cnt) {
+                        atomicAdd(&kp.results[2 * g], cnt);
+                        atomicAdd(&kp.results[2 * g + 1], hs);
+                    }
+                }
+                if (lane() == 0) {
+                    atomicAdd(pending, ~0ull);
+                    if (kp.progress) {
+                        __hip_atomic_store(&kp.progress[2 * wid], roots + ovf, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_SYSTEM);
+                        __hip_atomic_store(&kp.progress[2 * wid + 1], nodes, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_SYSTEM);
+                    }
+                }
+                continue;
+            }
+            roots_left = false;
+        }
+        if (kp.n_slots) {
+            const int64_t pos = dq_claim_pop(kp);
+            if (pos >= 0) {
+                if (idle && lane() == 0) atomicAdd(kp.dq + 2, ~0ull);
+                idle = false;
+                backoff = 1;
+                const u64* slot =
+                    (const u64*)kp.slots + (size_t)((u64)pos % kp.n_slots) * kp.slot_words;
+                const uint32_t g = (uint32_t)slot[0];
+                Local S;
+                sec.mark(kSecWait);
+                const bool ok = slot[1] != 0 && load_snapshot(kp, S, arena, lds_mem, slot);
+                dq_release(kp, (u64)pos);
+                sec.mark(kSecFetch);
+                if (ok) {
+                    const DevGraph G = kp.graphs[g];
+                    u64 cnt = 0, hs = 0;
+                    stolen++;
+                    dfs<V>(kp, G, g, S, arena, lds_mem, premin, cnt, hs, nodes, donated, sec);
+                    if (lane() == 0 && cnt) {
+                        atomicAdd(&kp.results[2 * g], cnt);
+                        atomicAdd(&kp.results[2 * g + 1], hs);
+                    }
+                } else {
+                    dropped++;
+                }
+                if (lane() == 0) {
+                    atomicAdd(pending, ~0ull);
+                    if (kp.progress)
+                        __hip_atomic_store(&kp.progress[2 * wid + 1], nodes, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_SYSTEM);
+                }
+                continue;
             }
         }
+        if (!idle) {
+            if (lane() == 0) atomicAdd(kp.dq + 2, 1ull);
+            idle = true;
+        }
+        u64 pend = 0;
+        if (lane() == 0) pend = ld_acq(pending);
+        pend = uni64(pend);
+        if (pend == 0) break;
+        // exponential backoff keeps idle waves off the deque's cache lines
+        for (uint32_t i = 0; i < backoff; i++) __builtin_amdgcn_s_sleep(32);
+        backoff = min(backoff * 2, 32u);
+        sec.mark(kSecWait);
     }
+    sec.mark(kSecWait);
+    if (idle && lane() == 0) atomicAdd(kp.dq + 2, ~0ull);
     if (lane() == 0) {
         atomicAdd(&kp.stats[kStatNodes], nodes);
         atomicAdd(&kp.stats[kStatRoots], roots);
+        atomicAdd(&kp.stats[kStatDonated], donated);
+        atomicAdd(&kp.stats[kStatStolen], stolen);
         atomicAdd(&kp.stats[kStatOverflow], ovf);
         atomicAdd(&kp.stats[kStatBuildWords], build);
+        atomicAdd(&kp.stats[kStatDropped], dropped);
+        if (kp.sections)
+            for (int i = 0; i < kSecCount; i++) atomicAdd(&kp.sections[i], sec.t[i]);
     }
+    if (lane() == 0) {
+    }
+}
+
+// Reset the deque control block: head, tail, idle, pending, slot sequences.
+__global__ void mbe_dq_init(unsigned long long* dq, uint32_t n_slots) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 4) dq[i] = 0;
+    if (i < n_slots) dq[4 + i] = i;
 }
 
 }  // namespace
@@ -633,11 +1279,16 @@ hipError_t launch_mbe(const KParams& kp, uint32_t workers, hipStream_t stream) {
     return hipGetLastError();
 }
 
+hipError_t launch_dq_init(unsigned long long* dq, uint32_t n_slots, hipStream_t stream) {
+    const uint32_t n = n_slots > 4 ? n_slots : 4;
+    hipLaunchKernelGGL(mbe_dq_init, dim3((n + 255) / 256), dim3(256), 0, stream, dq, n_slots);
+    return hipGetLastError();
+}
+
 hipError_t set_lds_limit(uint32_t lds_words) {
     const int bytes = (int)(lds_words * sizeof(u64));
-    hipError_t e = hipSuccess;
-    e = hipFuncSetAttribute((const void*)mbe_persistent<kCuMBE>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    hipError_t e = hipFuncSetAttribute((const void*)mbe_persistent<kCuMBE>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     if (e != hipSuccess) return e;
     e = hipFuncSetAttribute((const void*)mbe_persistent<kNoRS>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
@@ -648,7 +1299,6 @@ hipError_t set_lds_limit(uint32_t lds_words) {
     return hipFuncSetAttribute((const void*)mbe_persistent<kNoWS>,
                                hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
 }
-
 #endif  // MBE_EMULATE
 
 }  // namespace mbe

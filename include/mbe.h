@@ -60,9 +60,13 @@ typedef struct {
     uint32_t list_cutoff;    /* P/Q size at or below which nodes use row
                                 (sorted-list style) scans instead of
                                 column bitmaps; 0 = default */
-    uint32_t donate_depth;   /* deepest level a worker donates from */
+    uint32_t donate_depth;   /* deepest level a worker donates from; 0 = any */
+    uint32_t compact_ratio;  /* re-localise a child subproblem when its universe
+                                shrank by this factor; 0 = default (2),
+                                UINT32_MAX = never */
+    uint32_t donate_interval;/* nodes between a worker's donation checks; 0 = 64 */
     int32_t side;            /* -1 reference side choice, 0 enumerate U, 1 V */
-    uint32_t flags;          /* reserved, 0 */
+    uint32_t flags;          /* MBE_FLAG_* */
 } mbe_tuning;
 
 typedef struct {
@@ -74,7 +78,16 @@ typedef struct {
     uint64_t donated;      /* subtree tasks donated through the deque */
     uint32_t launches;     /* kernel launches used */
     uint32_t workers;      /* workers used */
+    /* Time per kernel section summed over workers, in 10 ns ticks, filled
+     * when tuning->flags has MBE_FLAG_SECTIONS (the reference's DESECTION
+     * build, src/header.cuh:25-32): other, root build, candidate selection,
+     * L' construction, maximality check (non-maximal), check + expansion
+     * (maximal), report, re-localisation, donation, stolen-subtree fetch,
+     * idle wait. */
+    uint64_t sections[11];
 } mbe_result;
+
+#define MBE_FLAG_SECTIONS 1u
 
 const char* mbe_status_str(mbe_status s);
 int mbe_last_hip_error(void);       /* hipError_t of the last MBE_EHIP */

@@ -68,7 +68,7 @@ def test_small_arena_defers_roots():
     """Roots whose local problem exceeds the arena are deferred to a retry pass."""
     g = G.random_graph(120, 90, 0.2, 5)
     ref = O.solve(*g)
-    r = M.solve(gpu_graph(g), "cuMBE", {"arena_words": 2150, "workers": 16})
+    r = M.solve(gpu_graph(g), "cuMBE", {"arena_words": 300, "lds_words": 420, "workers": 16})
     assert (r.count, r.hash) == (ref.count, ref.hash)
     assert r.launches >= 2
 
@@ -100,3 +100,18 @@ def test_batch_matches_per_graph_oracle():
     for i, (r, ref) in enumerate(zip(per, refs)):
         assert (r.count, r.hash) == (ref.count, ref.hash), i
     assert tot.count == sum(r.count for r in refs)
+
+
+def test_work_stealing_donates():
+    """Subtrees move through the HBM deque; results stay bit-exact."""
+    g = G.crown(14)
+    ref = O.solve(*g)
+    assert ref.count == 2 ** 14 - 2
+    for v in ("cuMBE", "noRS", "noES"):
+        say(f"stealing variant {v}")
+        r = M.solve(gpu_graph(g), v, {"workers": 8})
+        assert (r.count, r.hash) == (ref.count, ref.hash), v
+        assert r.donated > 0, v
+    r = M.solve(gpu_graph(g), "noWS", {"workers": 8})
+    assert (r.count, r.hash) == (ref.count, ref.hash)
+    assert r.donated == 0
