@@ -48,6 +48,11 @@ class Tuning(C.Structure):
         ("donate_interval", C.c_uint32),
         ("side", C.c_int32),
         ("flags", C.c_uint32),
+        ("root_begin", C.c_uint32),
+        ("root_end", C.c_uint32),
+        ("root_stride", C.c_uint32),
+        ("max_nodes", C.c_uint64),
+        ("time_limit_ms", C.c_double),
     ]
 
 
@@ -62,6 +67,8 @@ class Result(C.Structure):
         ("launches", C.c_uint32),
         ("workers", C.c_uint32),
         ("sections", C.c_uint64 * 11),
+        ("words", C.c_uint64),
+        ("build_words", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:
@@ -210,11 +217,19 @@ def _tuning(tuning: Optional[dict]) -> Optional[Tuning]:
     return t
 
 
-def solve(graph: Graph, variant: str = "cuMBE", tuning: Optional[dict] = None) -> Result:
-    """Enumerate all maximal bicliques of `graph` on the GPU (one launch per graph)."""
+def solve(graph: Graph, variant: str = "cuMBE", tuning: Optional[dict] = None,
+          allow_partial: bool = False) -> Result:
+    """Enumerate all maximal bicliques of `graph` on the GPU (one launch per graph).
+
+    With a watchdog in `tuning` (max_nodes / time_limit_ms) and allow_partial,
+    a run the watchdog stopped returns its partial result with r.partial set.
+    """
     r = Result()
     t = _tuning(tuning)
-    _check(lib().mbe_solve(graph.handle, VARIANTS[variant], C.byref(t) if t else None, C.byref(r)))
+    st = lib().mbe_solve(graph.handle, VARIANTS[variant], C.byref(t) if t else None, C.byref(r))
+    r.partial = st == -4
+    if not (allow_partial and r.partial):
+        _check(st)
     return r
 
 

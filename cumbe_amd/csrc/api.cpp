@@ -81,6 +81,7 @@ struct Engine {
     uint32_t lds_set = 0;
     unsigned long long* progress = nullptr;  // host-mapped, 2 per worker
     size_t progress_slots = 0;
+    unsigned long long* abort_flag = nullptr;  // host-mapped watchdog flag
 
     mbe_status ensure_init() {
         if (init) return MBE_OK;
@@ -95,6 +96,7 @@ struct Engine {
         HIPC(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         HIPC(hipEventCreate(&ev0));
         HIPC(hipEventCreate(&ev1));
+        HIPC(hipHostMalloc((void**)&abort_flag, 64, hipHostMallocMapped | hipHostMallocCoherent));
         init = true;
         return MBE_OK;
     }
@@ -110,6 +112,9 @@ struct Tune {
     uint64_t arena_words;
     int side;
     uint32_t flags;
+    uint32_t root_begin, root_end, root_stride;
+    uint64_t max_nodes;
+    double time_limit_ms;
 };
 
 Tune resolve(const mbe_tuning* t, int cus, size_t lds_per_cu) {
@@ -127,6 +132,11 @@ Tune resolve(const mbe_tuning* t, int cus, size_t lds_per_cu) {
     r.compact_ratio = (t && t->compact_ratio) ? (t->compact_ratio == UINT32_MAX ? 0 : t->compact_ratio) : 2;
     r.side = t ? t->side : -1;
     r.flags = t ? t->flags : 0;
+    r.root_begin = t ? t->root_begin : 0;
+    r.root_end = t ? t->root_end : 0;
+    r.root_stride = (t && t->root_stride) ? t->root_stride : 1;
+    r.max_nodes = t ? t->max_nodes : 0;
+    r.time_limit_ms = t ? t->time_limit_ms : 0;
     r.donate_interval = (t && t->donate_interval) ? t->donate_interval : 64;
     return r;
 }
@@ -181,7 +191,8 @@ mbe_status upload(Engine& E, const std::vector<mbe::Prepared>& ps, std::vector<m
 
 // Wait for the launch; with MBE_PROGRESS=<seconds> set, print the roots and
 // nodes the workers have finished (read from host-mapped slots) to stderr.
-hipError_t wait_with_progress(Engine& E, uint32_t nw, uint64_t n_tasks, int pass) {
+hipError_t wait_with_progress(Engine& E, uint32_t nw, uint64_t n_tasks, int pass,
+                              double deadline_ms) {
     const char* env = getenv("MBE_PROGRESS");
     const double every = env ? atof(env) * 1000.0 : 0.0;
     const double t0 = now_ms();
@@ -190,6 +201,8 @@ hipError_t wait_with_progress(Engine& E, uint32_t nw, uint64_t n_tasks, int pass
         hipError_t q = hipEventQuery(E.ev1);
         if (q != hipErrorNotReady) return q == hipSuccess ? hipEventSynchronize(E.ev1) : q;
         usleep(2000);
+        if (deadline_ms > 0 && now_ms() > deadline_ms)
+            __atomic_store_n(E.abort_flag, 1ull, __ATOMIC_SEQ_CST);
         if (every > 0 && now_ms() - last >= every) {
             last = now_ms();
             unsigned long long roots = 0, nodes = 0;
@@ -232,6 +245,11 @@ mbe_status run_tasks(Engine& E, const std::vector<mbe::Prepared>& ps,
     uint64_t arena_words = tu.arena_words;
     uint32_t workers = tu.workers;
     std::vector<uint64_t> task_list;  // empty = range mode
+    if (ng == 1 && (tu.root_begin || tu.root_end || tu.root_stride > 1)) {
+        const uint64_t end = tu.root_end ? std::min<uint64_t>(tu.root_end, total_tasks) : total_tasks;
+        for (uint64_t r = tu.root_begin; r < end; r += tu.root_stride) task_list.push_back(r);
+        n_tasks = task_list.size();
+    }
     float kernel_ms = 0;
     uint32_t launches = 0;
     const uint64_t ovf_cap = std::max<uint64_t>(total_tasks, 1);
@@ -239,6 +257,8 @@ mbe_status run_tasks(Engine& E, const std::vector<mbe::Prepared>& ps,
     uint64_t* ovf_list = E.ovf.as<uint64_t>();
     uint64_t* task_dev = ovf_list + ovf_cap;
     const uint64_t budget_words = (uint64_t)48 << 30 >> 3;  // 48 GiB of arenas
+    const double deadline = tu.time_limit_ms > 0 ? now_ms() + tu.time_limit_ms : 0;
+    __atomic_store_n(E.abort_flag, 0ull, __ATOMIC_SEQ_CST);
     for (int pass = 0; n_tasks > 0; pass++) {
         if (pass > 6) return fail(MBE_ENOMEM, "root subproblem does not fit any arena");
         HIPC(E.arena.ensure(arena_words * workers * 8));
@@ -251,7 +271,9 @@ mbe_status run_tasks(Engine& E, const std::vector<mbe::Prepared>& ps,
         }
         HIPC(hipMemsetAsync(cursor, 0, 16, E.stream));
         const uint32_t n_slots = 1024;
-        const uint64_t slot_words = std::min<uint64_t>(std::max<uint64_t>(4096, arena_words / 8), 1u << 16);
+        // a slot holds a compacted level of any problem that fits an arena,
+        // up to 4 MiB per slot (1024 slots)
+        const uint64_t slot_words = std::min<uint64_t>(std::max<uint64_t>(4096, arena_words), 1u << 19);
         HIPC(E.slots.ensure((size_t)n_slots * slot_words * 8));
         HIPC(E.dq.ensure((4 + (size_t)n_slots) * 8));
         HIPC(mbe::launch_dq_init(E.dq.as<unsigned long long>(), n_slots, E.stream));
@@ -274,6 +296,9 @@ mbe_status run_tasks(Engine& E, const std::vector<mbe::Prepared>& ps,
         kp.list_cutoff = tu.list_cutoff;
         kp.compact_ratio = tu.compact_ratio;
         kp.sections = (tu.flags & MBE_FLAG_SECTIONS) ? secs : nullptr;
+        HIPC(hipHostGetDevicePointer((void**)&kp.abort, E.abort_flag, 0));
+        kp.node_count = stats + mbe::kStatNodeBudget;
+        kp.node_budget = tu.max_nodes;
         kp.overflow_count = ovf_count;
         kp.overflow_list = ovf_list;
         kp.overflow_cap = ovf_cap;
@@ -298,7 +323,7 @@ mbe_status run_tasks(Engine& E, const std::vector<mbe::Prepared>& ps,
         HIPC(hipEventRecord(E.ev0, E.stream));
         HIPC(mbe::launch_mbe(kp, nw, E.stream));
         HIPC(hipEventRecord(E.ev1, E.stream));
-        HIPC(wait_with_progress(E, nw, n_tasks, pass));
+        HIPC(wait_with_progress(E, nw, n_tasks, pass, deadline));
         float ms = 0;
         HIPC(hipEventElapsedTime(&ms, E.ev0, E.ev1));
         kernel_ms += ms;
@@ -306,6 +331,7 @@ mbe_status run_tasks(Engine& E, const std::vector<mbe::Prepared>& ps,
         out.workers = std::max(out.workers, nw);
         unsigned long long novf = 0;
         HIPC(hipMemcpy(&novf, ovf_count, 8, hipMemcpyDeviceToHost));
+        if (__atomic_load_n(E.abort_flag, __ATOMIC_SEQ_CST)) break;
         if (!novf) break;
         task_list.resize(novf);
         HIPC(hipMemcpy(task_list.data(), ovf_list, novf * 8, hipMemcpyDeviceToHost));
@@ -320,11 +346,15 @@ mbe_status run_tasks(Engine& E, const std::vector<mbe::Prepared>& ps,
     for (int i = 0; i < mbe::kSecCount; i++) out.sections[i] = sc[i];
     HIPC(hipMemcpy(rr.data(), results, rr.size() * 8, hipMemcpyDeviceToHost));
     res.assign(rr.begin(), rr.end());
-    if (st[mbe::kStatDropped]) return fail(MBE_ENOMEM, "donated subtree did not fit a worker arena");
     out.kernel_ms = kernel_ms;
     out.nodes = st[mbe::kStatNodes];
     out.donated = st[mbe::kStatDonated];
     out.launches = launches;
+    out.words = st[mbe::kStatWords];
+    out.build_words = st[mbe::kStatBuildWords];
+    if (__atomic_load_n(E.abort_flag, __ATOMIC_SEQ_CST))
+        return fail(MBE_ETIMEOUT, "watchdog: node budget or time limit reached; results partial");
+    if (st[mbe::kStatDropped]) return fail(MBE_ENOMEM, "donated subtree did not fit a worker arena");
     return MBE_OK;
 }
 
@@ -453,7 +483,7 @@ mbe_status mbe_batch_solve(const mbe_graph* const* graphs, size_t n, mbe_variant
         s = upload(E, ps, dg, total_tasks);
         if (s != MBE_OK) return s;
         s = run_tasks(E, ps, dg, total_tasks, (int)variant, tu, res, agg);
-        if (s != MBE_OK) return s;
+        if (s != MBE_OK && s != MBE_ETIMEOUT) return s;  // a timeout keeps partial results
     }
     agg.total_ms = now_ms() - t0;
     for (size_t i = 0; i < n; i++) {
@@ -471,7 +501,7 @@ mbe_status mbe_batch_solve(const mbe_graph* const* graphs, size_t n, mbe_variant
             per_graph[i].total_ms = agg.total_ms;
         }
     if (total) *total = agg;
-    return MBE_OK;
+    return s;
 }
 
 mbe_status mbe_solve(const mbe_graph* g, mbe_variant variant, const mbe_tuning* tuning,
@@ -480,7 +510,7 @@ mbe_status mbe_solve(const mbe_graph* g, mbe_variant variant, const mbe_tuning* 
     const mbe_graph* one[1] = {g};
     mbe_result r{};
     mbe_status s = mbe_batch_solve(one, 1, variant, tuning, nullptr, &r);
-    if (s == MBE_OK) *out = r;
+    if (s == MBE_OK || s == MBE_ETIMEOUT) *out = r;
     return s;
 }
 

@@ -30,7 +30,9 @@ enum KStat : int {
     kStatOverflow = 4,   // root tasks deferred for a larger arena
     kStatBuildWords = 5, // adjacency entries read while building local problems
     kStatDropped = 6,    // slots discarded (snapshot did not fit the thief's arena)
-    kStatCount = 8
+    kStatNodeBudget = 7, // nodes charged against the watchdog budget
+    kStatWords = 8,      // bit-matrix operand words the nodes read (algorithmic)
+    kStatCount = 12
 };
 
 // Kernel time sections (DESECTION equivalent).
@@ -89,6 +91,10 @@ struct KParams {
     // host-mapped progress: 2 slots per worker (roots done, nodes), plain stores
     unsigned long long* progress;
     unsigned long long* sections;   // kSecCount tick sums, or null
+    // watchdog: abort flag (host-mapped, system scope) and node budget
+    unsigned long long* abort;
+    unsigned long long* node_count;
+    uint64_t node_budget;           // 0 = unlimited
 };
 
 }  // namespace mbe

@@ -15,7 +15,9 @@
 //   rows[k]  = N(cand k)  n N(x)   as a d-bit set      (forward view)
 //   cols[j]  = N(u_j) n cands      as an nc-bit set    (reverse view)
 // A frame at level k holds x, the running R hash sum, |R|, |P| and |Q|, then
-// L_k (W words), P_k and Q_k (Wc words each). When everything fits, the whole
+// L_k (W words), P_k, Q_k and X_k (Wc words each). X_k holds candidates that
+// follow P_k in this level's order but were handed to another worker: they
+// take part in expansion (absorption, P') but are never selected here. When everything fits, the whole
 // problem (hash terms, both bit matrices and all frames) lives in LDS;
 // otherwise the frames take the LDS and the rest sits in the worker's HBM
 // arena. All accesses go through generic pointers.
@@ -138,6 +140,26 @@ __device__ __forceinline__ u64 red_or64(u64 v) {
 
 __device__ __forceinline__ void wsync() { __syncthreads(); }
 
+// Watchdog: the host (time limit) or any worker (node budget) raises
+// *abort; workers then drop their work and drain the deque. Checked every
+// kAbortEvery nodes.
+constexpr uint32_t kAbortEvery = 256;
+__device__ __forceinline__ bool aborted(const KParams& kp) {
+    if (!kp.abort) return false;
+    u64 a = 0;
+    if (lane() == 0) a = __hip_atomic_load(kp.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    return uni64(a) != 0;
+}
+__device__ __forceinline__ bool charge_nodes(const KParams& kp, u64 n) {
+    u64 total = 0;
+    if (lane() == 0 && kp.node_budget) {
+        total = atomicAdd(kp.node_count, n) + n;
+        if (total >= kp.node_budget)
+            __hip_atomic_store(kp.abort, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+    return aborted(kp);
+}
+
 // Per-section time accounting (the reference's DESECTION clocks,
 // src/header.cuh:25-32), enabled when KParams::sections is set. Ticks are
 // wall_clock64() units (100 MHz) summed over workers.
@@ -215,7 +237,7 @@ __device__ bool layout(const KParams& kp, Local& S, u64* arena, u64* lds, size_t
     S.W = (S.d + 63) >> 6;
     S.Wc = (S.nc + 63) >> 6;
     S.maxdepth = min(S.d, S.nc) + 2;
-    S.fs = kFrameHdr + S.W + 2 * S.Wc;
+    S.fs = kFrameHdr + S.W + 3 * S.Wc;
     S.lds_off = lds_off;
     S.base = off;
     const size_t data = (size_t)S.d + S.nc + (size_t)S.nc * S.W + (size_t)S.d * S.Wc;
@@ -383,8 +405,10 @@ __device__ uint32_t make_lprime(const Local& S, const u64* L, uint32_t x, u64* L
 __device__ NodeOut process_rs(const Local& S, u64* F, u64* Fc, uint32_t nl) {
     const u64* P = F + kFrameHdr + S.W;
     const u64* Q = P + S.Wc;
+    const u64* X = Q + S.Wc;
     u64* Pc = Fc + kFrameHdr + S.W;
     u64* Qc = Pc + S.Wc;
+    u64* Xc = Qc + S.Wc;
     NodeOut o{true, false, 0, 0, 0, 0, 0};
     u64 abh = 0;
     if (S.Wc <= kColVecMax) {
@@ -401,7 +425,7 @@ __device__ NodeOut process_rs(const Local& S, u64* F, u64* Fc, uint32_t nl) {
                 a &= red_and64(ca);
                 ob |= red_or64(co);
             }
-            const u64 q = uni64(Q[w]), p = uni64(P[w]);
+            const u64 q = uni64(Q[w]), p = uni64(P[w] | X[w]);
             if (a & q) {
                 o.maximal = false;
                 break;
@@ -410,6 +434,7 @@ __device__ NodeOut process_rs(const Local& S, u64* F, u64* Fc, uint32_t nl) {
             if (lane() == 0) {
                 Pc[w] = pn;
                 Qc[w] = qn;
+                Xc[w] = 0;
             }
             o.ab += __popcll(absd);
             o.pn += __popcll(pn);
@@ -443,12 +468,13 @@ __device__ NodeOut process_rs(const Local& S, u64* F, u64* Fc, uint32_t nl) {
                     ob |= c;
                 }
             }
-            const u64 q = valid ? Q[w] : 0, p = valid ? P[w] : 0;
+            const u64 q = valid ? Q[w] : 0, p = valid ? (P[w] | X[w]) : 0;
             nonmax |= (a & q) != 0;
             const u64 absd = a & p, pn = ob & p & ~a, qn = ob & q;
             if (valid) {
                 Pc[w] = pn;
                 Qc[w] = qn;
+                Xc[w] = 0;
             }
             ab += __popcll(absd);
             pnc += __popcll(pn);
@@ -478,9 +504,11 @@ __device__ NodeOut process_rs(const Local& S, u64* F, u64* Fc, uint32_t nl) {
 __device__ NodeOut process_fwd(const Local& S, u64* F, u64* Fc, uint32_t nl) {
     const u64* P = F + kFrameHdr + S.W;
     const u64* Q = P + S.Wc;
+    const u64* X = Q + S.Wc;
     const u64* Lc = Fc + kFrameHdr;
     u64* Pc = Fc + kFrameHdr + S.W;
     u64* Qc = Pc + S.Wc;
+    u64* Xc = Qc + S.Wc;
     NodeOut o{true, false, 0, 0, 0, 0, 0};
     for (uint32_t w = 0; w < S.Wc; w++) {
         const u64 qw = uni64(Q[w]);
@@ -513,7 +541,7 @@ __device__ NodeOut process_fwd(const Local& S, u64* F, u64* Fc, uint32_t nl) {
     wsync();
     u64 abh = 0;
     for (uint32_t w = 0; w < S.Wc; w++) {
-        const u64 pw = uni64(P[w]);
+        const u64 pw = uni64(P[w] | X[w]);
         u64 pn = 0, absd = 0;
         if (pw) {
             const bool valid = (pw >> lane()) & 1;
@@ -526,7 +554,10 @@ __device__ NodeOut process_fwd(const Local& S, u64* F, u64* Fc, uint32_t nl) {
             pn = __ballot(valid && c != 0 && c != nl);
             if ((absd >> lane()) & 1) abh += S.hc[w * 64 + lane()];
         }
-        if (lane() == 0) Pc[w] = pn;
+        if (lane() == 0) {
+            Pc[w] = pn;
+            Xc[w] = 0;
+        }
         o.ab += __popcll(absd);
         o.pn += __popcll(pn);
         o.un += __popcll(pn | uni64(Qc[w]));
@@ -616,39 +647,54 @@ __device__ void dq_release(const KParams& kp, u64 pos) {
     if (lane() == 0) st_rel(kp.dq + 4 + (pos % kp.n_slots), pos + kp.n_slots);
 }
 
-// Count |L| and |P u Q (u x)| of a frame, for sizing a compaction.
-__device__ __forceinline__ void frame_sizes(const Local& S, const u64* L, const u64* P, const u64* Q,
-                                            uint32_t xq, uint32_t& dn, uint32_t& n) {
-    uint32_t a = 0, b = 0;
-    for (uint32_t w = lane(); w < S.W; w += 64) a += __popcll(L[w]);
-    for (uint32_t w = lane(); w < S.Wc; w += 64) {
-        u64 cw = P[w] | Q[w];
-        if (w == (xq >> 6)) cw |= 1ull << (xq & 63);
-        b += __popcll(cw);
+// Source sets of a compaction: the candidates of P at index >= t stay in P,
+// those below t join Q (they precede the handed-over part), X stays X, and
+// candidate xq (the level's in-progress x, or none) joins Q.
+struct SplitSrc {
+    const u64 *L, *P, *Q, *X;
+    uint32_t t, xq;
+    __device__ __forceinline__ u64 pw(uint32_t w) const {
+        const u64 m = w * 64 >= t ? ~0ull : (w * 64 + 64 <= t ? 0ull : ~0ull << (t & 63));
+        return P[w] & m;
     }
+    __device__ __forceinline__ u64 qw(uint32_t w) const {
+        u64 v = Q[w] | (P[w] & ~pw_mask(w));
+        if (w == (xq >> 6)) v |= 1ull << (xq & 63);
+        return v;
+    }
+    __device__ __forceinline__ u64 pw_mask(uint32_t w) const {
+        return w * 64 >= t ? ~0ull : (w * 64 + 64 <= t ? 0ull : ~0ull << (t & 63));
+    }
+    __device__ __forceinline__ u64 cw(uint32_t w) const { return pw(w) | qw(w) | X[w]; }
+};
+
+// |L| and the number of candidates of a compaction source.
+__device__ __forceinline__ void frame_sizes(const Local& S, const SplitSrc& src, uint32_t& dn,
+                                            uint32_t& n) {
+    uint32_t a = 0, b = 0;
+    for (uint32_t w = lane(); w < S.W; w += 64) a += __popcll(src.L[w]);
+    for (uint32_t w = lane(); w < S.Wc; w += 64) b += __popcll(src.cw(w));
     dn = red_add32(a);
     n = red_add32(b);
 }
 
-// Compact (L, P, Q [+ x into Q]) of problem S into the arrays of a smaller
-// universe: L's set bits renumbered 0..dn-1, the candidates of P u Q (u x)
-// renumbered 0..n-1 in order. Writes hl', hc', rows' (dn-bit rows), P', Q'.
-// S.list and S.lbits are overwritten (scratch).
-__device__ void compact_sets(const Local& S, const u64* L, const u64* P, const u64* Q, uint32_t xq,
-                             u64* hl2, u64* hc2, u64* rows2, u64* P2, u64* Q2, uint32_t dn,
-                             uint32_t n) {
+// Compact a source of problem S into the arrays of a smaller universe: L's
+// set bits renumbered 0..dn-1, the candidates renumbered 0..n-1 in order.
+// Writes hl', hc', rows' (dn-bit rows), P', Q', X'. S.list and S.lbits are
+// overwritten (scratch).
+__device__ void compact_sets(const Local& S, const SplitSrc& src, u64* hl2, u64* hc2, u64* rows2,
+                             u64* P2, u64* Q2, u64* X2, uint32_t dn, uint32_t n) {
     const uint32_t Wn = (dn + 63) >> 6;
     // candidate list (ascending) and L bit list
     uint32_t n0 = 0;
     for (uint32_t w = 0; w < S.Wc; w++) {
-        u64 cw = uni64(P[w] | Q[w]);
-        if (w == (xq >> 6)) cw |= 1ull << (xq & 63);
+        const u64 cw = uni64(src.cw(w));
         if ((cw >> lane()) & 1) S.list[n0 + __popcll(cw & lanemask_lt())] = w * 64 + lane();
         n0 += __popcll(cw);
     }
     uint32_t d0 = 0;
     for (uint32_t w = 0; w < S.W; w++) {
-        const u64 word = uni64(L[w]);
+        const u64 word = uni64(src.L[w]);
         if ((word >> lane()) & 1) {
             const uint32_t at = d0 + __popcll(word & lanemask_lt());
             S.lbits[at] = w * 64 + lane();
@@ -657,7 +703,7 @@ __device__ void compact_sets(const Local& S, const u64* L, const u64* P, const u
         d0 += __popcll(word);
     }
     wsync();
-    // candidates: hash terms, rows bit-extracted over L, P'/Q' by ballot
+    // candidates: hash terms, rows bit-extracted over L, P'/Q'/X' by ballot
     for (uint32_t c0 = 0; c0 < n; c0 += 64) {
         const uint32_t kn = c0 + lane();
         const bool valid = kn < n;
@@ -673,39 +719,41 @@ __device__ void compact_sets(const Local& S, const u64* L, const u64* P, const u
             }
             if (valid) rows2[(size_t)kn * Wn + w2] = acc;
         }
-        const bool inP = valid && ((P[k >> 6] >> (k & 63)) & 1);
-        const bool inQ = valid && ((((Q[k >> 6] >> (k & 63)) & 1)) || k == xq);
-        const u64 bp = __ballot(inP), bq = __ballot(inQ);
+        const uint32_t w = k >> 6, b = k & 63;
+        const bool inP = valid && ((src.pw(w) >> b) & 1);
+        const bool inQ = valid && ((src.qw(w) >> b) & 1);
+        const bool inX = valid && ((src.X[w] >> b) & 1);
+        const u64 bp = __ballot(inP), bq = __ballot(inQ), bx = __ballot(inX);
         if (lane() == 0) {
             P2[c0 >> 6] = bp;
             Q2[c0 >> 6] = bq;
+            X2[c0 >> 6] = bx;
         }
     }
     wsync();
 }
 
-// Write the compaction of level k (L_k, remaining P_k, Q_k u {x_k}) into a
-// slot as a LEVEL call over its own universe.
-__device__ void write_snapshot(const KParams& kp, const Local& S, u64* F, uint32_t g, u64* slot) {
-    const u64* L = F + kFrameHdr;
-    const u64* P = L + S.W;
-    const u64* Q = P + S.Wc;
-    const uint32_t xk = (uint32_t)F[0];
+// Write a compaction source into a slot as a LEVEL call over its own
+// universe: [0] graph, [1] d, [2] nc, [3] R hash sum, [4] |R|, then hl[d],
+// hc[nc], rows[nc * W], P[Wc], Q[Wc], X[Wc].
+__device__ void write_snapshot(const Local& S, const SplitSrc& src, u64 sr, u64 nr, uint32_t g,
+                               u64* slot) {
     uint32_t dn, n;
-    frame_sizes(S, L, P, Q, xk, dn, n);
+    frame_sizes(S, src, dn, n);
     const uint32_t Wn = (dn + 63) >> 6, Wcn = (n + 63) >> 6;
     u64* hl2 = slot + 8;
     u64* hc2 = hl2 + dn;
     u64* rows2 = hc2 + n;
     u64* P2 = rows2 + (size_t)n * Wn;
     u64* Q2 = P2 + Wcn;
-    compact_sets(S, L, P, Q, xk, hl2, hc2, rows2, P2, Q2, dn, n);
+    u64* X2 = Q2 + Wcn;
+    compact_sets(S, src, hl2, hc2, rows2, P2, Q2, X2, dn, n);
     if (lane() == 0) {
         slot[0] = g;
         slot[1] = dn;
         slot[2] = n;
-        slot[3] = F[1];
-        slot[4] = F[2];
+        slot[3] = sr;
+        slot[4] = nr;
     }
 }
 
@@ -725,6 +773,7 @@ __device__ bool load_snapshot(const KParams& kp, Local& S, u64* arena, u64* lds,
     const u64* rows2 = hc2 + S.nc;
     const u64* P2 = rows2 + (size_t)S.nc * S.W;
     const u64* Q2 = P2 + S.Wc;
+    const u64* X2 = Q2 + S.Wc;
     for (uint32_t i = lane(); i < S.d; i += 64) S.hl[i] = hl2[i];
     for (uint32_t i = lane(); i < S.nc; i += 64) S.hc[i] = hc2[i];
     const size_t nr = (size_t)S.nc * S.W;
@@ -735,11 +784,13 @@ __device__ bool load_snapshot(const KParams& kp, Local& S, u64* arena, u64* lds,
     u64* L1 = F1 + kFrameHdr;
     u64* P1 = L1 + S.W;
     u64* Q1 = P1 + S.Wc;
+    u64* X1 = Q1 + S.Wc;
     fill_full_l(S, L1);
     uint32_t pc = 0, qc = 0;
     for (uint32_t w = lane(); w < S.Wc; w += 64) {
         P1[w] = P2[w];
         Q1[w] = Q2[w];
+        X1[w] = X2[w];
         pc += __popcll(P2[w]);
         qc += __popcll(Q2[w]);
     }
@@ -774,8 +825,10 @@ __device__ bool compact_child(const KParams& kp, const Local& S, uint32_t k, con
     u64* L1 = F1 + kFrameHdr;
     u64* P1 = L1 + S2.W;
     u64* Q1 = P1 + S2.Wc;
+    u64* X1 = Q1 + S2.Wc;
     const u64 sr = Fc[1], nr = Fc[2];
-    compact_sets(S, L, P, Q, 0xFFFFFFFFu, S2.hl, S2.hc, S2.rows, P1, Q1, dn, n);
+    const SplitSrc src{L, P, Q, Q + S.Wc, 0, 0xFFFFFFFFu};
+    compact_sets(S, src, S2.hl, S2.hc, S2.rows, P1, Q1, X1, dn, n);
     build_cols(S2);
     fill_full_l(S2, L1);
     if (lane() == 0) {
@@ -787,15 +840,19 @@ __device__ bool compact_child(const KParams& kp, const Local& S, uint32_t k, con
     return true;
 }
 
-// Size in words of the snapshot of frame F (x_k joins Q).
-__device__ size_t snapshot_words(const Local& S, const u64* F) {
-    const u64* L = F + kFrameHdr;
-    const u64* P = L + S.W;
-    const u64* Q = P + S.Wc;
+// Size in words of a snapshot of the given source, or ~0 when the snapshot
+// or the problem it becomes would not fit a slot or a thief's arena.
+__device__ size_t snapshot_words(const KParams& kp, const Local& S, const SplitSrc& src) {
     uint32_t dn, n;
-    frame_sizes(S, L, P, Q, (uint32_t)F[0], dn, n);
-    const uint32_t Wn = (dn + 63) >> 6, Wcn = (n + 63) >> 6;
-    return dn ? 8 + (size_t)dn + n + (size_t)n * Wn + 2 * (size_t)Wcn : ~(size_t)0;
+    frame_sizes(S, src, dn, n);
+    if (!dn) return ~(size_t)0;
+    const size_t Wn = (dn + 63) >> 6, Wcn = (n + 63) >> 6;
+    const size_t words = 8 + (size_t)dn + n + (size_t)n * Wn + 3 * Wcn;
+    // the thief's layout with nothing in LDS (see layout())
+    const size_t thief = (size_t)dn + n + (size_t)n * Wn + (size_t)dn * Wcn + (n + 1) / 2 + 1 +
+                         (dn + 1) / 2 + 1 + (size_t)(min(dn, n) + 2) * (kFrameHdr + Wn + 3 * Wcn);
+    if (words > kp.slot_words || kArenaHdr + thief > kp.arena_words) return ~(size_t)0;
+    return words;
 }
 
 // Rebuild a suspended problem from its nest record.
@@ -809,10 +866,30 @@ __device__ __forceinline__ void unpack(const KParams& kp, const u64* rec, Local&
     depth0 = (uint32_t)rec[3];
 }
 
-// Donate the shallowest level that still has unselected candidates and
-// whose snapshot fits a slot: suspended outer problems first (levels up to
-// the one their nested child hangs from), then the current problem's levels
-// below `cur`. Returns true when a subtree was handed over.
+// Index t of the (h+1)-th set bit of P (so h bits lie below t).
+__device__ uint32_t split_point(const Local& S, const u64* P, uint32_t h) {
+    uint32_t seen = 0;
+    for (uint32_t w = 0; w < S.Wc; w++) {
+        const u64 pw = uni64(P[w]);
+        const uint32_t c = __popcll(pw);
+        if (seen + c > h) {
+            u64 m = pw;
+            for (uint32_t i = seen; i < h; i++) m &= m - 1;
+            return w * 64 + __builtin_ctzll(m);
+        }
+        seen += c;
+    }
+    return S.Wc * 64;
+}
+
+// Hand part of a level to the deque: the shallowest level (suspended outer
+// problems first, then the current one) that still has unselected work.
+// A level whose x is in progress gives away its remaining P whole when it
+// holds one candidate, else its upper half; the current level (no x yet)
+// gives its upper half when it holds two or more. The donor keeps the lower
+// part A and moves the given part B into X; the thief gets P = B,
+// Q = Q u A (u x), X = X: the order "x, A, B, X" is a valid level order, so
+// both sides enumerate exactly the nodes of the sequential walk.
 __device__ bool try_donate(const KParams& kp, const Local& S, uint32_t cur, uint32_t g,
                            u64* arena, u64* lds, const u64* nest, uint32_t np) {
     for (uint32_t i = 0; i <= np; i++) {
@@ -822,13 +899,22 @@ __device__ bool try_donate(const KParams& kp, const Local& S, uint32_t cur, uint
             unpack(kp, nest + 4 * i, T, arena, lds, top, d0);
         } else {
             T = S;
-            top = cur - 1;
+            top = cur;
         }
         if (kp.donate_depth) top = min(top, kp.donate_depth);
         for (uint32_t k = 1; k <= top; k++) {
             u64* F = T.frame(k);
-            if (pcount(F) == 0) continue;
-            if (snapshot_words(T, F) > kp.slot_words) continue;
+            const uint32_t pc = pcount(F);
+            const bool running = i < np || k < cur;  // x[k] in progress
+            if (pc == 0 || (!running && pc < 2)) continue;
+            u64* L = F + kFrameHdr;
+            u64* P = L + T.W;
+            u64* Q = P + T.Wc;
+            u64* X = Q + T.Wc;
+            const uint32_t keep = pc / 2;  // A: the lowest `keep` candidates
+            const uint32_t t = keep ? split_point(T, P, keep) : 0;
+            const SplitSrc src{L, P, Q, X, t, running ? (uint32_t)F[0] : 0xFFFFFFFFu};
+            if (snapshot_words(kp, T, src) == ~(size_t)0) continue;
             if (lane() == 0) atomicAdd(kp.dq + 3, 1ull);  // pending, before publishing
             int64_t pos = dq_claim_push(kp);
             if (pos < 0) {
@@ -836,11 +922,14 @@ __device__ bool try_donate(const KParams& kp, const Local& S, uint32_t cur, uint
                 return false;
             }
             u64* slot = (u64*)kp.slots + (size_t)((u64)pos % kp.n_slots) * kp.slot_words;
-            write_snapshot(kp, T, F, g, slot);
+            write_snapshot(T, src, F[1], F[2], g, slot);
             dq_publish(kp, (u64)pos);
-            u64* P = F + kFrameHdr + T.W;
-            for (uint32_t w = lane(); w < T.Wc; w += 64) P[w] = 0;
-            if (lane() == 0) F[3] &= ~0xFFFFFFFFull;  // |P| = 0
+            for (uint32_t w = lane(); w < T.Wc; w += 64) {
+                const u64 b = src.pw(w);
+                X[w] |= b;
+                P[w] &= ~b;
+            }
+            if (lane() == 0) F[3] = (F[3] & ~0xFFFFFFFFull) | keep;  // |P| = |A|
             wsync();
             return true;
         }
@@ -855,11 +944,16 @@ __device__ bool try_donate(const KParams& kp, const Local& S, uint32_t cur, uint
 template <int V>
 __device__ void dfs(const KParams& kp, const DevGraph& G, uint32_t g, Local& S, u64* arena,
                     u64* lds, uint32_t* premin, u64& cnt, u64& hs, u64& nodes, u64& donated,
-                    Sec& sec) {
+                    u64& words, Sec& sec) {
     u64* nest = lds + kLdsNest;
     uint32_t np = 0, depth0 = 0, k = 1;
     u64 next_check = nodes + kp.donate_interval;
+    u64 next_abort = nodes + kAbortEvery;
     for (;;) {
+        if (nodes >= next_abort) {
+            next_abort = nodes + kAbortEvery;
+            if (charge_nodes(kp, kAbortEvery)) break;
+        }
         u64* F = S.frame(k);
         const u64 cnts = uni64(F[3]);
         const uint32_t pc = (uint32_t)cnts, qc = (uint32_t)(cnts >> 32);
@@ -907,9 +1001,12 @@ __device__ void dfs(const KParams& kp, const DevGraph& G, uint32_t g, Local& S, 
         u64* Fc = S.frame(k + 1);
         const uint32_t nl = make_lprime(S, L, xk, Fc + kFrameHdr);
         sec.mark(kSecLPrime);
-        const NodeOut o = (V == kNoRS || pc - 1 + qc <= kp.list_cutoff)
-                              ? process_fwd(S, F, Fc, nl)
-                              : process_rs(S, F, Fc, nl);
+        const bool fwd = V == kNoRS || pc - 1 + qc <= kp.list_cutoff;
+        const NodeOut o = fwd ? process_fwd(S, F, Fc, nl) : process_rs(S, F, Fc, nl);
+        // operand words of this node: selection rows and P words, x's row
+        // and L, the L' columns (or the P and Q rows) with P/Q/X, L' terms
+        words += (u64)pc * S.W + S.Wc + 2 * S.W + nl +
+                 (fwd ? (u64)(pc - 1 + qc) * S.W + 3 * S.Wc : (u64)S.Wc * (nl + 3));
         sec.mark(o.maximal ? kSecExpand : kSecCheck);
         if (o.maximal) {
             const u64 sl = sum_l(S, nl);
@@ -1073,7 +1170,9 @@ __device__ int build_root(const KParams& kp, const DevGraph& G, uint32_t x, u64*
     u64* L1 = F1 + kFrameHdr;
     u64* P1 = L1 + S.W;
     u64* Q1 = P1 + S.Wc;
+    u64* X1 = Q1 + S.Wc;
     fill_full_l(S, L1);
+    for (uint32_t w = lane(); w < S.Wc; w += 64) X1[w] = 0;
     bool nonmax = false;
     uint32_t ab = 0;
     u64 abh = 0;
@@ -1127,7 +1226,7 @@ __global__ __launch_bounds__(64) void mbe_persistent(KParams kp) {
     uint32_t* cpos = kp.cpos + (size_t)wid * kp.cpos_len;
     uint32_t* premin = (uint32_t*)(lds_mem + kLdsPreMin);
     for (uint32_t i = lane(); i < kPreMinLevels; i += 64) premin[i] = 1;  // src/main.cu:86
-    u64 nodes = 0, roots = 0, build = 0, ovf = 0, donated = 0, stolen = 0, dropped = 0;
+    u64 nodes = 0, roots = 0, build = 0, ovf = 0, donated = 0, stolen = 0, dropped = 0, words = 0;
     bool roots_left = true, idle = false;
     uint32_t backoff = 1;
     unsigned long long* pending = kp.dq + 3;
@@ -1135,6 +1234,8 @@ __global__ __launch_bounds__(64) void mbe_persistent(KParams kp) {
     sec.init(kp.sections != nullptr, lds_mem);
     for (;;) {
         wsync();
+        const bool stop = aborted(kp);
+        if (stop) roots_left = false;
         if (roots_left) {
             u64 t = 0;
             if (lane() == 0) {
@@ -1156,7 +1257,8 @@ __global__ __launch_bounds__(64) void mbe_persistent(KParams kp) {
                 const int st = build_root<V>(kp, G, x, arena, lds_mem, mark, cpos, S, cnt, hs,
                                              nodes, build);
                 sec.mark(kSecBuild);
-                if (st > 0) dfs<V>(kp, G, g, S, arena, lds_mem, premin, cnt, hs, nodes, donated, sec);
+                if (st > 0)
+                    dfs<V>(kp, G, g, S, arena, lds_mem, premin, cnt, hs, nodes, donated, words, sec);
                 if (st < 0) {
                     ovf++;
                     if (lane() == 0) {
@@ -1194,14 +1296,17 @@ __global__ __launch_bounds__(64) void mbe_persistent(KParams kp) {
                 const uint32_t g = (uint32_t)slot[0];
                 Local S;
                 sec.mark(kSecWait);
-                const bool ok = slot[1] != 0 && load_snapshot(kp, S, arena, lds_mem, slot);
+                const bool ok = !stop && slot[1] != 0 && load_snapshot(kp, S, arena, lds_mem, slot);
                 dq_release(kp, (u64)pos);
                 sec.mark(kSecFetch);
-                if (ok) {
+                if (stop) {
+                    // watchdog: discard the subtree
+                } else if (ok) {
                     const DevGraph G = kp.graphs[g];
                     u64 cnt = 0, hs = 0;
                     stolen++;
-                    dfs<V>(kp, G, g, S, arena, lds_mem, premin, cnt, hs, nodes, donated, sec);
+                    dfs<V>(kp, G, g, S, arena, lds_mem, premin, cnt, hs, nodes, donated, words,
+                           sec);
                     if (lane() == 0 && cnt) {
                         atomicAdd(&kp.results[2 * g], cnt);
                         atomicAdd(&kp.results[2 * g + 1], hs);
@@ -1241,6 +1346,7 @@ __global__ __launch_bounds__(64) void mbe_persistent(KParams kp) {
         atomicAdd(&kp.stats[kStatOverflow], ovf);
         atomicAdd(&kp.stats[kStatBuildWords], build);
         atomicAdd(&kp.stats[kStatDropped], dropped);
+        atomicAdd(&kp.stats[kStatWords], words);
         if (kp.sections)
             for (int i = 0; i < kSecCount; i++) atomicAdd(&kp.sections[i], sec.t[i]);
     }

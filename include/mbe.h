@@ -67,6 +67,16 @@ typedef struct {
     uint32_t donate_interval;/* nodes between a worker's donation checks; 0 = 64 */
     int32_t side;            /* -1 reference side choice, 0 enumerate U, 1 V */
     uint32_t flags;          /* MBE_FLAG_* */
+    /* Root subset: ranks r with root_begin <= r < root_end (0 = all) and
+     * (r - root_begin) % root_stride == 0 (0 = 1). Interleaving roots with a
+     * stride shards one graph across devices or processes: the partial
+     * counts and hashes of the shards add up to the whole. Single graph only. */
+    uint32_t root_begin, root_end, root_stride;
+    /* Watchdog: stop after about max_nodes tree nodes or time_limit_ms of
+     * wall time (0 = none); the solve then returns MBE_ETIMEOUT and the
+     * result holds the partial count, hash and node total. */
+    uint64_t max_nodes;
+    double time_limit_ms;
 } mbe_tuning;
 
 typedef struct {
@@ -85,6 +95,11 @@ typedef struct {
      * (maximal), report, re-localisation, donation, stolen-subtree fetch,
      * idle wait. */
     uint64_t sections[11];
+    /* Algorithmic work: 64-bit words of the local bit matrices and set
+     * bitsets that the visited nodes read (from the set sizes, not counted by
+     * hardware), and adjacency entries read while localising roots. */
+    uint64_t words;
+    uint64_t build_words;
 } mbe_result;
 
 #define MBE_FLAG_SECTIONS 1u

@@ -106,6 +106,7 @@ typedef struct {
     mbe_oracle_emit_fn emit;
     void* user;
     atomic_llong cursor;           /* the reference's P_ptr, src/header.cuh:35 */
+    int64_t sub_begin, sub_end, sub_stride;  /* root subset by rank */
 } problem;
 
 typedef struct {
@@ -338,9 +339,16 @@ static void select_candidate(worker* w, int lvl) {
     w->v2P[tmp] = imin;
 }
 
+/* Claim the next root (decreasing index). Roots outside the requested
+ * subset are skipped; the Q sweep of the next claim moves them into Q
+ * exactly as roots claimed by other blocks are (src/mbe_noWS.cuh:135-150). */
 static int64_t claim_root(problem* pb) {
-    long long i = atomic_fetch_sub(&pb->cursor, 1);
-    return i >= 0 ? (int64_t)i : -1;
+    for (;;) {
+        long long i = atomic_fetch_sub(&pb->cursor, 1);
+        if (i < 0) return -1;
+        if (i >= pb->sub_begin && i < pb->sub_end && (i - pb->sub_begin) % pb->sub_stride == 0)
+            return (int64_t)i;
+    }
 }
 
 /* one reference block's phase-1 loop, serially: src/mbe_noWS.cuh:101-615 */
@@ -398,7 +406,8 @@ static double now_s(void) {
 
 static int solve_impl(uint32_t n_u, uint32_t n_v, const uint32_t* off_u, const uint32_t* adj_u,
                       int rs, int es, int side, int threads, mbe_oracle_emit_fn emit,
-                      void* user, mbe_oracle_result* out) {
+                      void* user, int64_t rb, int64_t re, int64_t rstride,
+                      mbe_oracle_result* out) {
     if (!out || (!off_u && n_u) || threads < 1) return -1;
     memset(out, 0, sizeof(*out));
     if (emit && threads != 1) return -1;
@@ -417,6 +426,9 @@ static int solve_impl(uint32_t n_u, uint32_t n_v, const uint32_t* off_u, const u
     pb.offL = Ls->off; pb.adjL = Ls->adj;
     pb.r_is_u = r_is_u;
     pb.rs = rs; pb.es = es; pb.emit = emit; pb.user = user;
+    pb.sub_begin = rb;
+    pb.sub_end = re > 0 ? re : (int64_t)1 << 62;
+    pb.sub_stride = rstride > 0 ? rstride : 1;
     pb.hR = malloc(sizeof(uint64_t) * ((size_t)pb.nR + 1));
     pb.hL = malloc(sizeof(uint64_t) * ((size_t)pb.nL + 1));
     pb.ori_P = malloc(sizeof(uint32_t) * ((size_t)pb.nR + 1));
@@ -467,13 +479,22 @@ done:
 
 int mbe_oracle_solve(uint32_t n_u, uint32_t n_v, const uint32_t* off_u, const uint32_t* adj_u,
                      int rs, int es, int side, int threads, mbe_oracle_result* out) {
-    return solve_impl(n_u, n_v, off_u, adj_u, rs, es, side, threads, NULL, NULL, out);
+    return solve_impl(n_u, n_v, off_u, adj_u, rs, es, side, threads, NULL, NULL, 0, 0, 1, out);
+}
+
+int mbe_oracle_solve_roots(uint32_t n_u, uint32_t n_v, const uint32_t* off_u,
+                           const uint32_t* adj_u, int rs, int es, int side, int threads,
+                           int64_t root_begin, int64_t root_end, int64_t root_stride,
+                           mbe_oracle_result* out) {
+    if (root_begin < 0 || root_stride < 0) return -1;
+    return solve_impl(n_u, n_v, off_u, adj_u, rs, es, side, threads, NULL, NULL, root_begin,
+                      root_end, root_stride, out);
 }
 
 int mbe_oracle_solve_emit(uint32_t n_u, uint32_t n_v, const uint32_t* off_u,
                           const uint32_t* adj_u, int rs, int es, int side,
                           mbe_oracle_emit_fn emit, void* user, mbe_oracle_result* out) {
-    return solve_impl(n_u, n_v, off_u, adj_u, rs, es, side, 1, emit, user, out);
+    return solve_impl(n_u, n_v, off_u, adj_u, rs, es, side, 1, emit, user, 0, 0, 1, out);
 }
 
 /* ------------------------------------------------------------ brute force */

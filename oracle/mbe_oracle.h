@@ -71,6 +71,15 @@ int mbe_oracle_solve(uint32_t n_u, uint32_t n_v, const uint32_t* off_u,
                      const uint32_t* adj_u, int rs, int es, int side, int threads,
                      mbe_oracle_result* out);
 
+/* Same, restricted to the roots of rank r (root order: degree descending,
+ * id ascending) with root_begin <= r < root_end (0 = no bound) and
+ * (r - root_begin) % root_stride == 0. The results of a partition of the
+ * roots add up to the whole graph's. */
+int mbe_oracle_solve_roots(uint32_t n_u, uint32_t n_v, const uint32_t* off_u,
+                           const uint32_t* adj_u, int rs, int es, int side, int threads,
+                           int64_t root_begin, int64_t root_end, int64_t root_stride,
+                           mbe_oracle_result* out);
+
 int mbe_oracle_solve_emit(uint32_t n_u, uint32_t n_v, const uint32_t* off_u,
                           const uint32_t* adj_u, int rs, int es, int side,
                           mbe_oracle_emit_fn emit, void* user,

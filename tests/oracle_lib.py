@@ -38,6 +38,10 @@ def lib() -> C.CDLL:
         L.mbe_oracle_solve_emit.argtypes = [C.c_uint32, C.c_uint32, P, P, C.c_int, C.c_int,
                                             C.c_int, EMIT, C.c_void_p, C.POINTER(OracleResult)]
         L.mbe_oracle_solve_emit.restype = C.c_int
+        L.mbe_oracle_solve_roots.argtypes = [C.c_uint32, C.c_uint32, P, P, C.c_int, C.c_int,
+                                             C.c_int, C.c_int, C.c_int64, C.c_int64, C.c_int64,
+                                             C.POINTER(OracleResult)]
+        L.mbe_oracle_solve_roots.restype = C.c_int
         L.mbe_oracle_brute.argtypes = [C.c_uint32, C.c_uint32, P, P, C.POINTER(OracleResult)]
         L.mbe_oracle_brute.restype = C.c_int
         L.mbe_oracle_biclique_hash.argtypes = [P, C.c_uint32, P, C.c_uint32]
@@ -57,6 +61,18 @@ def solve(n_u, n_v, off, adj, rs=1, es=1, side=-1, threads=1) -> OracleResult:
     off, adj, po, pa = _arrs(off, adj)
     r = OracleResult()
     rc = lib().mbe_oracle_solve(n_u, n_v, po, pa, rs, es, side, threads, C.byref(r))
+    if rc:
+        raise ValueError(f"oracle rejected input ({rc})")
+    return r
+
+
+def solve_roots(n_u, n_v, off, adj, begin=0, end=0, stride=1, rs=1, es=1, side=-1,
+                threads=1) -> OracleResult:
+    """Oracle restricted to the roots of rank in [begin, end) step stride."""
+    off, adj, po, pa = _arrs(off, adj)
+    r = OracleResult()
+    rc = lib().mbe_oracle_solve_roots(n_u, n_v, po, pa, rs, es, side, threads, begin, end,
+                                      stride, C.byref(r))
     if rc:
         raise ValueError(f"oracle rejected input ({rc})")
     return r

@@ -115,3 +115,16 @@ def test_work_stealing_donates():
     r = M.solve(gpu_graph(g), "noWS", {"workers": 8})
     assert (r.count, r.hash) == (ref.count, ref.hash)
     assert r.donated == 0
+
+
+def test_watchdog_stops_cleanly():
+    """A node budget stops the kernel early with a partial result; the next
+    solve on the same engine is complete and exact."""
+    g = G.crown(16)
+    r = M.solve(gpu_graph(g), "cuMBE", {"max_nodes": 2000}, allow_partial=True)
+    assert r.partial and r.count < 2 ** 16 - 2
+    with pytest.raises(M.MbeError):
+        M.solve(gpu_graph(g), "cuMBE", {"max_nodes": 2000})
+    ref = O.solve(*G.crown(10))
+    r = M.solve(gpu_graph(G.crown(10)), "cuMBE")
+    assert not r.partial and (r.count, r.hash) == (ref.count, ref.hash)
