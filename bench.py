@@ -38,7 +38,8 @@ HBM_PEAK_GBS = 8000.0
 WORKLOAD = dict(n_u=94000, n_v=30000, m=293000, gamma_u=2.0, gamma_v=1.8, max_deg=7600, seed=2)
 ROOT_BEGIN = 400
 SAMPLE = dict(begin=1000, end=0, stride=64)
-TUNING = dict(lds_words=2048, waves_per_cu=8, donate_interval=256)
+TUNING_KEY = "c1_roots400"
+TUNING = dict(lds_words=4096, waves_per_cu=4, donate_interval=64)
 
 
 def traffic_from_profiles():
@@ -93,7 +94,9 @@ def main():
     g = M.Graph.chung_lu(WORKLOAD["n_u"], WORKLOAD["n_v"], WORKLOAD["m"], WORKLOAD["gamma_u"],
                          WORKLOAD["gamma_v"], WORKLOAD["max_deg"], WORKLOAD["seed"])
     n_u, n_v, n_e = g.info()
-    tun = dict(TUNING, root_begin=args.root_begin)
+    from cumbe_amd import autotune
+    base = autotune.cached(TUNING_KEY) or TUNING
+    tun = dict(base, root_begin=args.root_begin)
     # engine initialisation (device, code objects) outside any step
     M.solve(M.Graph.chung_lu(200, 100, 600, 2.1, 2.1, 0, 1), args.variant)
 
@@ -154,7 +157,7 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
         },
-        "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(g, TUNING),
+        "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(g, base),
     }
     print(json.dumps(line))
 

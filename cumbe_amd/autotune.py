@@ -1,0 +1,116 @@
+"""Autotuner for the enumeration launch parameters.
+
+Picks, per workload key, the per-worker LDS budget (which fixes how many
+single-wave workers a CU holds), the workers per CU, the donation interval of
+the work-stealing deque, the list-versus-bitmap cutoff (|P|+|Q| at or below
+which a node tests candidate rows instead of AND/OR-ing L' columns) and the
+re-localisation ratio. Coordinate descent from a start point; every trial
+runs under the engine's watchdog (a time limit), and a trial that hits it is
+discarded. Results are cached in tuning_cache.json next to this file and
+returned by cached(key).
+
+Usage: python -m cumbe_amd.autotune <workload> [--budget-s S]
+  workloads: c1_roots400 (the bench workload), batch_sample (32 graphs of the
+  batched workload's distribution)
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from typing import Callable, Dict, List, Optional
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CACHE = os.path.join(HERE, "tuning_cache.json")
+
+GRID = {
+    "lds_words": [2048, 3072, 4096, 6144, 8192],
+    "waves_per_cu": [2, 3, 4, 5, 6, 8],
+    "donate_interval": [16, 64, 256, 1024],
+    "list_cutoff": [0, 16, 64, 256],
+    "compact_ratio": [2, 4, 0xFFFFFFFF],
+}
+START = {"lds_words": 4096, "waves_per_cu": 4, "donate_interval": 64, "list_cutoff": 0,
+         "compact_ratio": 2}
+
+
+def cached(key: str) -> Optional[dict]:
+    if not os.path.exists(CACHE):
+        return None
+    with open(CACHE) as f:
+        return json.load(f).get(key, {}).get("tuning")
+
+
+def _store(key: str, tuning: dict, ms: float, trials: List[dict]) -> None:
+    data = {}
+    if os.path.exists(CACHE):
+        with open(CACHE) as f:
+            data = json.load(f)
+    data[key] = {"tuning": tuning, "kernel_ms": ms, "trials": trials,
+                 "when": time.strftime("%Y-%m-%d %H:%M:%S")}
+    with open(CACHE, "w") as f:
+        json.dump(data, f, indent=1, sort_keys=True)
+
+
+def tune(key: str, run: Callable[[dict], Optional[float]], budget_s: float,
+         start: Optional[dict] = None) -> dict:
+    """Coordinate descent: run(tuning) returns kernel ms (None if it failed)."""
+    best = dict(start or START)
+    best_ms = run(best)
+    trials = [{"tuning": dict(best), "ms": best_ms}]
+    t0 = time.time()
+    improved = True
+    while improved and time.time() - t0 < budget_s:
+        improved = False
+        for name, values in GRID.items():
+            for v in values:
+                if v == best[name] or time.time() - t0 > budget_s:
+                    continue
+                cand = dict(best, **{name: v})
+                ms = run(cand)
+                trials.append({"tuning": cand, "ms": ms})
+                print(f"autotune {key}: {name}={v} -> {ms}", flush=True)
+                if ms is not None and (best_ms is None or ms < best_ms * 0.97):
+                    best, best_ms, improved = cand, ms, True
+    _store(key, best, best_ms, trials)
+    return best
+
+
+def _workload(key: str):
+    sys.path.insert(0, os.path.dirname(HERE))
+    import cumbe_amd as M
+
+    if key == "c1_roots400":
+        g = M.Graph.chung_lu(94000, 30000, 293000, 2.0, 1.8, 7600, 2)
+
+        def run(t):
+            r = M.solve(g, "cuMBE", dict(t, root_begin=400, time_limit_ms=60000),
+                        allow_partial=True)
+            return None if r.partial else r.kernel_ms
+        return run
+    if key == "batch_sample":
+        gs = [M.Graph.batch_graph(1000 + 128 * i) for i in range(32)]
+
+        def run(t):
+            try:
+                per, tot = M.batch_solve(gs, "cuMBE", dict(t, time_limit_ms=60000))
+            except M.MbeError:
+                return None
+            return tot.kernel_ms
+        return run
+    raise SystemExit(f"unknown workload {key}")
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("workload")
+    ap.add_argument("--budget-s", type=float, default=240)
+    args = ap.parse_args()
+    best = tune(args.workload, _workload(args.workload), args.budget_s)
+    print(json.dumps({"workload": args.workload, "tuning": best}))
+
+
+if __name__ == "__main__":
+    main()

@@ -328,33 +328,51 @@ __device__ uint32_t select_x(const Local& S, u64* F, uint32_t* premin_lvl) {
     const uint32_t pm = *premin_lvl;
     uint32_t best = kInf, arg = 0;
     bool nostop = (V == kNoES);
-    for (int w = (int)S.Wc - 1; w >= 0; w--) {
-        const u64 pw = uni64(P[w]);
-        if (!pw) continue;
-        const bool valid = (pw >> lane()) & 1;
-        uint32_t c = kInf;
-        if (valid) {
-            c = 0;
-            const u64* r = S.rows + (size_t)(w * 64 + lane()) * S.W;
-            for (uint32_t i = 0; i < S.W; i++) c += __popcll(r[i] & L[i]);
+    bool done = false;
+    // four P words per step so that their row gathers are in flight together;
+    // the ES scan then walks them in descending order
+    for (int w0 = (int)S.Wc - 1; w0 >= 0 && !done; w0 -= 4) {
+        u64 pw[4];
+        uint32_t c[4];
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            const int w = w0 - s;
+            pw[s] = w >= 0 ? uni64(P[w]) : 0;
+            c[s] = kInf;
         }
-        if (!nostop) {
-            const u64 m = __ballot(valid && c <= pm);
-            if (m) {
-                const uint32_t s = 63 - __builtin_clzll(m);
-                const uint32_t cs = rdlane32(c, s);
-                if (cs == pm) {
-                    best = cs;
-                    arg = w * 64 + s;
-                    break;
-                }
-                nostop = true;
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            if ((pw[s] >> lane()) & 1) {
+                const u64* r = S.rows + (size_t)((w0 - s) * 64 + lane()) * S.W;
+                uint32_t cc = 0;
+                for (uint32_t i = 0; i < S.W; i++) cc += __popcll(r[i] & L[i]);
+                c[s] = cc;
             }
         }
-        const uint32_t cm = red_min32(c);
-        if (cm < best) {
-            best = cm;
-            arg = w * 64 + (63 - __builtin_clzll(__ballot(c == cm)));
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            if (done || !pw[s]) continue;
+            const int w = w0 - s;
+            const bool valid = (pw[s] >> lane()) & 1;
+            if (!nostop) {
+                const u64 m = __ballot(valid && c[s] <= pm);
+                if (m) {
+                    const uint32_t sl = 63 - __builtin_clzll(m);
+                    const uint32_t cs = rdlane32(c[s], sl);
+                    if (cs == pm) {
+                        best = cs;
+                        arg = w * 64 + sl;
+                        done = true;
+                        continue;
+                    }
+                    nostop = true;
+                }
+            }
+            const uint32_t cm = red_min32(c[s]);
+            if (cm < best) {
+                best = cm;
+                arg = w * 64 + (63 - __builtin_clzll(__ballot(c[s] == cm)));
+            }
         }
     }
     if (lane() == 0) {
