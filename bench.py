@@ -47,7 +47,7 @@ def traffic_from_profiles():
     path = os.path.join(ROOT, "profiles", "mbe_persistent_c1_roots400.pmc.txt")
     if not os.path.exists(path):
         return None
-    m = re.search(r"^hbm_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read(), re.M)
+    m = re.search(r"^\s*hbm_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read(), re.M)
     return float(m.group(1)) if m else None
 
 

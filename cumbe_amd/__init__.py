@@ -112,9 +112,14 @@ def lib() -> C.CDLL:
     L.mbe_gen_batch_graph.argtypes = [C.c_uint64, P(G)]
     L.mbe_solve.argtypes = [G, C.c_int, P(Tuning), P(Result)]
     L.mbe_batch_solve.argtypes = [P(G), C.c_size_t, C.c_int, P(Tuning), P(Result), P(Result)]
+    L.mbe_batch_plan_create.argtypes = [P(G), C.c_size_t, C.c_int, P(Tuning), P(G)]
+    L.mbe_batch_plan_run.argtypes = [G, P(Result), P(Result)]
+    L.mbe_batch_plan_free.argtypes = [G]
+    L.mbe_batch_plan_free.restype = None
     for name in ("mbe_graph_load_bi", "mbe_graph_write_bi", "mbe_graph_from_edge_list",
                  "mbe_graph_from_csr", "mbe_graph_info", "mbe_graph_csr", "mbe_gen_chung_lu",
-                 "mbe_gen_er_planted", "mbe_gen_batch_graph", "mbe_solve", "mbe_batch_solve"):
+                 "mbe_gen_er_planted", "mbe_gen_batch_graph", "mbe_solve", "mbe_batch_solve",
+                 "mbe_batch_plan_create", "mbe_batch_plan_run"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -233,12 +238,54 @@ def solve(graph: Graph, variant: str = "cuMBE", tuning: Optional[dict] = None,
     return r
 
 
-def batch_solve(graphs: List[Graph], variant: str = "cuMBE", tuning: Optional[dict] = None):
-    """All graphs in one persistent launch; returns (per-graph results, total)."""
+def batch_solve(graphs: List[Graph], variant: str = "cuMBE", tuning: Optional[dict] = None,
+                allow_partial: bool = False):
+    """All graphs in one persistent launch; returns (per-graph results, total).
+
+    With a watchdog in `tuning` and allow_partial, a stopped run returns its
+    partial results with tot.partial set."""
     n = len(graphs)
     arr = (C.c_void_p * max(n, 1))(*[g.handle.value for g in graphs])
     per = (Result * max(n, 1))()
     tot = Result()
     t = _tuning(tuning)
-    _check(lib().mbe_batch_solve(arr, n, VARIANTS[variant], C.byref(t) if t else None, per, C.byref(tot)))
+    st = lib().mbe_batch_solve(arr, n, VARIANTS[variant], C.byref(t) if t else None, per, C.byref(tot))
+    tot.partial = st == -4
+    if not (allow_partial and tot.partial):
+        _check(st)
     return list(per)[:n], tot
+
+
+class BatchPlan:
+    """Graphs resident on the device with their launch captured in a HIP graph;
+    run() replays it (mbe_batch_plan_*)."""
+
+    def __init__(self, graphs: List[Graph], variant: str = "cuMBE",
+                 tuning: Optional[dict] = None):
+        self.n = len(graphs)
+        arr = (C.c_void_p * max(self.n, 1))(*[g.handle.value for g in graphs])
+        h = C.c_void_p()
+        t = _tuning(tuning)
+        _check(lib().mbe_batch_plan_create(arr, self.n, VARIANTS[variant],
+                                           C.byref(t) if t else None, C.byref(h)))
+        self._h = h
+
+    def run(self, allow_partial: bool = False):
+        per = (Result * max(self.n, 1))()
+        tot = Result()
+        st = lib().mbe_batch_plan_run(self._h, per, C.byref(tot))
+        tot.partial = st == -4
+        if not (allow_partial and tot.partial):
+            _check(st)
+        return list(per)[:self.n], tot
+
+    def close(self) -> None:
+        if self._h:
+            lib().mbe_batch_plan_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -12,38 +12,27 @@
 #include <vector>
 #include <unistd.h>
 
-#include "../../include/mbe.h"
-#include "engine.h"
-#include "graph.h"
-
-namespace mbe {
-hipError_t launch_mbe(const KParams& kp, uint32_t workers, hipStream_t stream);
-hipError_t launch_dq_init(unsigned long long* dq, uint32_t n_slots, hipStream_t stream);
-hipError_t set_lds_limit(uint32_t lds_words);
-}  // namespace mbe
+#include "host.h"
 
 struct mbe_graph {
     mbe::Graph g;
 };
 
 namespace {
-
 thread_local std::string g_err;
 thread_local int g_hip_err = 0;
+}  // namespace
+
+namespace mbe {
+namespace host {
 
 mbe_status fail(mbe_status s, const std::string& msg) {
     g_err = msg;
     return s;
 }
 
-#define HIPC(expr)                                                                         \
-    do {                                                                                   \
-        hipError_t e_ = (expr);                                                            \
-        if (e_ != hipSuccess) {                                                            \
-            g_hip_err = (int)e_;                                                           \
-            return fail(MBE_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));      \
-        }                                                                                  \
-    } while (0)
+void set_hip_error(int e) { g_hip_err = e; }
+
 
 double now_ms() {
     return std::chrono::duration<double, std::milli>(
@@ -51,22 +40,7 @@ double now_ms() {
         .count();
 }
 
-// Device buffer that grows on demand and is reused across solves.
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    hipError_t ensure(size_t need) {
-        if (need <= bytes) return hipSuccess;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-        hipError_t e = hipMalloc(&p, need);
-        if (e == hipSuccess) bytes = need;
-        return e;
-    }
-    template <class T>
-    T* as() const { return (T*)p; }
-};
+
 
 // Process-wide engine state for device 0: worker resources are allocated once
 // and reused by every solve (a solve is serialised by the mutex).
@@ -77,10 +51,7 @@ struct Engine {
     size_t lds_per_cu = 160 * 1024;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    DevBuf arena, mark, cpos, ctl, graphs, data, ovf, slots, dq;
-    uint32_t lds_set = 0;
-    unsigned long long* progress = nullptr;  // host-mapped, 2 per worker
-    size_t progress_slots = 0;
+    Bufs b;
     unsigned long long* abort_flag = nullptr;  // host-mapped watchdog flag
 
     mbe_status ensure_init() {
@@ -107,15 +78,6 @@ Engine& engine() {
     return *e;
 }
 
-struct Tune {
-    uint32_t workers, lds_words, list_cutoff, donate_depth, compact_ratio, donate_interval;
-    uint64_t arena_words;
-    int side;
-    uint32_t flags;
-    uint32_t root_begin, root_end, root_stride;
-    uint64_t max_nodes;
-    double time_limit_ms;
-};
 
 Tune resolve(const mbe_tuning* t, int cus, size_t lds_per_cu) {
     Tune r;
@@ -142,8 +104,8 @@ Tune resolve(const mbe_tuning* t, int cus, size_t lds_per_cu) {
 }
 
 // Upload prepared graphs into one device buffer and build the descriptors.
-mbe_status upload(Engine& E, const std::vector<mbe::Prepared>& ps, std::vector<mbe::DevGraph>& dg,
-                  uint64_t& total_tasks) {
+mbe_status upload(Bufs& B, hipStream_t stream, const std::vector<mbe::Prepared>& ps,
+                  std::vector<mbe::DevGraph>& dg, uint64_t& total_tasks) {
     // one padding rule for sizing and placement: every array gets its bytes
     // plus 8 spare, rounded up to 256
     auto slot = [](size_t n) { return (n + 8 + 255) & ~(size_t)255; };
@@ -152,12 +114,12 @@ mbe_status upload(Engine& E, const std::vector<mbe::Prepared>& ps, std::vector<m
         bytes += slot(p.offR.size() * 4) + slot(p.adjR.size() * 4) + slot(p.offL.size() * 4) +
                  slot(p.adjL.size() * 4) + slot(p.hR.size() * 8) + slot(p.hL.size() * 8);
     }
-    HIPC(E.data.ensure(bytes));
+    HIPC(B.data.ensure(bytes));
     std::vector<char> host(bytes);
     size_t o = 0;
     dg.resize(ps.size());
     total_tasks = 0;
-    char* dbase = E.data.as<char>();
+    char* dbase = B.data.as<char>();
     auto put = [&](const void* src, size_t n) {
         size_t at = o;
         if (n) memcpy(host.data() + at, src, n);
@@ -181,17 +143,17 @@ mbe_status upload(Engine& E, const std::vector<mbe::Prepared>& ps, std::vector<m
         total_tasks += p.n_roots;
     }
     if (o != bytes) return fail(MBE_EINVAL, "upload layout mismatch");
-    HIPC(hipMemcpyAsync(E.data.p, host.data(), o, hipMemcpyHostToDevice, E.stream));
-    HIPC(E.graphs.ensure(sizeof(mbe::DevGraph) * std::max<size_t>(1, dg.size())));
-    HIPC(hipMemcpyAsync(E.graphs.p, dg.data(), sizeof(mbe::DevGraph) * dg.size(),
-                        hipMemcpyHostToDevice, E.stream));
-    HIPC(hipStreamSynchronize(E.stream));
+    HIPC(hipMemcpyAsync(B.data.p, host.data(), o, hipMemcpyHostToDevice, stream));
+    HIPC(B.graphs.ensure(sizeof(mbe::DevGraph) * std::max<size_t>(1, dg.size())));
+    HIPC(hipMemcpyAsync(B.graphs.p, dg.data(), sizeof(mbe::DevGraph) * dg.size(),
+                        hipMemcpyHostToDevice, stream));
+    HIPC(hipStreamSynchronize(stream));
     return MBE_OK;
 }
 
 // Wait for the launch; with MBE_PROGRESS=<seconds> set, print the roots and
 // nodes the workers have finished (read from host-mapped slots) to stderr.
-hipError_t wait_with_progress(Engine& E, uint32_t nw, uint64_t n_tasks, int pass,
+hipError_t wait_with_progress(Engine& E, const Bufs& B, uint32_t nw, uint64_t n_tasks, int pass,
                               double deadline_ms) {
     const char* env = getenv("MBE_PROGRESS");
     const double every = env ? atof(env) * 1000.0 : 0.0;
@@ -200,15 +162,15 @@ hipError_t wait_with_progress(Engine& E, uint32_t nw, uint64_t n_tasks, int pass
     for (;;) {
         hipError_t q = hipEventQuery(E.ev1);
         if (q != hipErrorNotReady) return q == hipSuccess ? hipEventSynchronize(E.ev1) : q;
-        usleep(2000);
+        usleep(200);
         if (deadline_ms > 0 && now_ms() > deadline_ms)
             __atomic_store_n(E.abort_flag, 1ull, __ATOMIC_SEQ_CST);
         if (every > 0 && now_ms() - last >= every) {
             last = now_ms();
             unsigned long long roots = 0, nodes = 0;
             for (uint32_t w = 0; w < nw; w++) {
-                roots += ((volatile unsigned long long*)E.progress)[2 * w];
-                nodes += ((volatile unsigned long long*)E.progress)[2 * w + 1];
+                roots += ((volatile unsigned long long*)B.progress)[2 * w];
+                nodes += ((volatile unsigned long long*)B.progress)[2 * w + 1];
             }
             fprintf(stderr, "[mbe] pass %d: %.1f s, %llu/%llu roots, %llu nodes\n", pass,
                     (last - t0) / 1000.0, roots, (unsigned long long)n_tasks, nodes);
@@ -216,135 +178,149 @@ hipError_t wait_with_progress(Engine& E, uint32_t nw, uint64_t n_tasks, int pass
     }
 }
 
+
+
+// Size the buffers of one pass and fill its kernel parameters. The task list
+// (explicit global task ids, or none for all tasks) is copied to the device.
+mbe_status setup_pass(Bufs& B, hipStream_t stream, unsigned long long* abort_host, const Meta& m,
+                      const Tune& tu, int variant, const std::vector<uint64_t>& task_list,
+                      uint64_t n_tasks, uint64_t arena_words, uint32_t workers, mbe::KParams& kp) {
+    HIPC(B.ctl.ensure(m.ctl_words() * 8));
+    const uint64_t ovf_cap = std::max<uint64_t>(m.total_tasks, 1);
+    HIPC(B.ovf.ensure(ovf_cap * 8 * 2));
+    HIPC(B.arena.ensure(arena_words * workers * 8));
+    HIPC(B.mark.ensure(m.mark_words() * workers * 8));
+    HIPC(B.cpos.ensure(m.cpos_len() * workers * 4));
+    // a slot holds a compacted level of any problem that fits an arena, up
+    // to 4 MiB per slot
+    const uint64_t slot_words = std::min<uint64_t>(std::max<uint64_t>(4096, arena_words), 1u << 19);
+    HIPC(B.slots.ensure((size_t)kSlots * slot_words * 8));
+    HIPC(B.dq.ensure((4 + (size_t)kSlots) * 8));
+    HIPC(B.ensure_progress(workers));
+    memset(B.progress, 0, 2 * (size_t)workers * 8);
+    unsigned long long* ctl = B.ctl.as<unsigned long long>();
+    uint64_t* ovf_list = B.ovf.as<uint64_t>();
+    uint64_t* task_dev = ovf_list + ovf_cap;
+    if (!task_list.empty())
+        HIPC(hipMemcpyAsync(task_dev, task_list.data(), task_list.size() * 8,
+                            hipMemcpyHostToDevice, stream));
+    kp = mbe::KParams{};
+    kp.graphs = B.graphs.as<mbe::DevGraph>();
+    kp.n_graphs = m.ng;
+    kp.variant = (uint32_t)variant;
+    kp.n_tasks = n_tasks;
+    kp.task_list = task_list.empty() ? nullptr : task_dev;
+    kp.cursor = ctl;
+    kp.overflow_count = ctl + 1;
+    kp.stats = ctl + 2;
+    kp.sections = (tu.flags & MBE_FLAG_SECTIONS) ? ctl + 2 + mbe::kStatCount : nullptr;
+    kp.results = ctl + 2 + mbe::kStatCount + mbe::kSecCount;
+    kp.arena = B.arena.as<uint64_t>();
+    kp.arena_words = arena_words;
+    kp.mark = B.mark.as<uint64_t>();
+    kp.mark_words = m.mark_words();
+    kp.cpos = B.cpos.as<uint32_t>();
+    kp.cpos_len = m.cpos_len();
+    kp.lds_words = tu.lds_words;
+    kp.list_cutoff = tu.list_cutoff;
+    kp.compact_ratio = tu.compact_ratio;
+    HIPC(hipHostGetDevicePointer((void**)&kp.abort, abort_host, 0));
+    kp.node_count = kp.stats + mbe::kStatNodeBudget;
+    kp.node_budget = tu.max_nodes;
+    kp.overflow_list = ovf_list;
+    kp.overflow_cap = ovf_cap;
+    kp.donate_depth = tu.donate_depth;
+    kp.donate_interval = tu.donate_interval;
+    kp.slots = B.slots.as<uint64_t>();
+    kp.slot_words = slot_words;
+    kp.n_slots = kSlots;
+    kp.dq = B.dq.as<unsigned long long>();
+    HIPC(hipHostGetDevicePointer((void**)&kp.progress, B.progress, 0));
+    return MBE_OK;
+}
+
+// Enqueue one pass with kernels only (so that it can be captured into a
+// graph): clear the control words (all of them on the first pass; cursor and
+// overflow count later, which keeps the results), clear the root marks,
+// reset the deque, launch the persistent workers.
+hipError_t enqueue_pass(const mbe::KParams& kp, uint32_t workers, size_t ctl_clear,
+                        hipStream_t s) {
+    hipError_t e = mbe::launch_fill(kp.cursor, ctl_clear, 0ull, s);
+    if (e == hipSuccess)
+        e = mbe::launch_fill((unsigned long long*)kp.mark, kp.mark_words * workers, 0ull, s);
+    if (e == hipSuccess) e = mbe::launch_dq_init(kp.dq, kp.n_slots, s);
+    if (e == hipSuccess) e = mbe::launch_mbe(kp, workers, s);
+    return e;
+}
+
+// A pass-0 launch prepared in advance (a batch plan's instantiated graph).
+struct Pass0 {
+    hipGraphExec_t exec = nullptr;
+    mbe::KParams kp{};
+    uint32_t workers = 0;
+};
+
 // Run every root task of the uploaded graphs; roots whose local problem does
 // not fit a worker arena are retried with fewer workers and larger arenas.
-mbe_status run_tasks(Engine& E, const std::vector<mbe::Prepared>& ps,
-                     const std::vector<mbe::DevGraph>& dg, uint64_t total_tasks, int variant,
-                     const Tune& tu, std::vector<uint64_t>& res, mbe_result& out) {
-    uint32_t max_nr = 1;
-    for (auto& p : ps) max_nr = std::max(max_nr, p.nR);
-    const uint64_t mark_words = (max_nr + 63) / 64 + 1;
-    const uint64_t cpos_len = max_nr + 1;
-    const size_t ng = dg.size();
-    // control block: cursor, overflow count, stats, results
-    const size_t ctl_words = 2 + mbe::kStatCount + mbe::kSecCount + 2 * ng;
-    HIPC(E.ctl.ensure(ctl_words * 8));
-    HIPC(hipMemsetAsync(E.ctl.p, 0, ctl_words * 8, E.stream));
-    unsigned long long* cursor = E.ctl.as<unsigned long long>();
-    unsigned long long* ovf_count = cursor + 1;
-    unsigned long long* stats = cursor + 2;
-    unsigned long long* secs = stats + mbe::kStatCount;
-    unsigned long long* results = secs + mbe::kSecCount;
-
-    if (E.lds_set < tu.lds_words) {
-        HIPC(mbe::set_lds_limit(tu.lds_words));
-        E.lds_set = tu.lds_words;
+mbe_status run_tasks(Engine& E, Bufs& B, const Meta& m, int variant, const Tune& tu,
+                     std::vector<uint64_t>& res, mbe_result& out, const Pass0* pre = nullptr) {
+    {
+        mbe_status s = ensure_lds(tu.lds_words);
+        if (s != MBE_OK) return s;
     }
-
-    uint64_t n_tasks = total_tasks;
+    uint64_t n_tasks = m.total_tasks;
     uint64_t arena_words = tu.arena_words;
     uint32_t workers = tu.workers;
     std::vector<uint64_t> task_list;  // empty = range mode
-    if (ng == 1 && (tu.root_begin || tu.root_end || tu.root_stride > 1)) {
-        const uint64_t end = tu.root_end ? std::min<uint64_t>(tu.root_end, total_tasks) : total_tasks;
+    if (m.ng == 1 && (tu.root_begin || tu.root_end || tu.root_stride > 1)) {
+        const uint64_t end = tu.root_end ? std::min<uint64_t>(tu.root_end, m.total_tasks) : m.total_tasks;
         for (uint64_t r = tu.root_begin; r < end; r += tu.root_stride) task_list.push_back(r);
         n_tasks = task_list.size();
     }
     float kernel_ms = 0;
     uint32_t launches = 0;
-    const uint64_t ovf_cap = std::max<uint64_t>(total_tasks, 1);
-    HIPC(E.ovf.ensure(ovf_cap * 8 * 2));
-    uint64_t* ovf_list = E.ovf.as<uint64_t>();
-    uint64_t* task_dev = ovf_list + ovf_cap;
     const uint64_t budget_words = (uint64_t)48 << 30 >> 3;  // 48 GiB of arenas
     const double deadline = tu.time_limit_ms > 0 ? now_ms() + tu.time_limit_ms : 0;
     __atomic_store_n(E.abort_flag, 0ull, __ATOMIC_SEQ_CST);
+    mbe::KParams kp{};
     for (int pass = 0; n_tasks > 0; pass++) {
         if (pass > 6) return fail(MBE_ENOMEM, "root subproblem does not fit any arena");
-        HIPC(E.arena.ensure(arena_words * workers * 8));
-        HIPC(E.mark.ensure(mark_words * workers * 8));
-        HIPC(E.cpos.ensure(cpos_len * workers * 4));
-        if (pass == 0) HIPC(hipMemsetAsync(E.mark.p, 0, mark_words * workers * 8, E.stream));
-        if (!task_list.empty()) {
-            HIPC(hipMemcpyAsync(task_dev, task_list.data(), task_list.size() * 8,
-                                hipMemcpyHostToDevice, E.stream));
+        uint32_t nw = workers;  // all start: those without a root task steal
+        if (pass == 0 && pre) {
+            kp = pre->kp;
+            nw = pre->workers;
+            memset(B.progress, 0, 2 * (size_t)nw * 8);
+            HIPC(hipEventRecord(E.ev0, E.stream));
+            HIPC(hipGraphLaunch(pre->exec, E.stream));
+        } else {
+            mbe_status s = setup_pass(B, E.stream, E.abort_flag, m, tu, variant, task_list, n_tasks,
+                                      arena_words, nw, kp);
+            if (s != MBE_OK) return s;
+            HIPC(hipEventRecord(E.ev0, E.stream));
+            HIPC(enqueue_pass(kp, nw, pass == 0 ? m.ctl_words() : 2, E.stream));
         }
-        HIPC(hipMemsetAsync(cursor, 0, 16, E.stream));
-        const uint32_t n_slots = 1024;
-        // a slot holds a compacted level of any problem that fits an arena,
-        // up to 4 MiB per slot (1024 slots)
-        const uint64_t slot_words = std::min<uint64_t>(std::max<uint64_t>(4096, arena_words), 1u << 19);
-        HIPC(E.slots.ensure((size_t)n_slots * slot_words * 8));
-        HIPC(E.dq.ensure((4 + (size_t)n_slots) * 8));
-        HIPC(mbe::launch_dq_init(E.dq.as<unsigned long long>(), n_slots, E.stream));
-        mbe::KParams kp{};
-        kp.graphs = E.graphs.as<mbe::DevGraph>();
-        kp.n_graphs = (uint32_t)ng;
-        kp.variant = (uint32_t)variant;
-        kp.n_tasks = n_tasks;
-        kp.task_list = task_list.empty() ? nullptr : task_dev;
-        kp.cursor = cursor;
-        kp.results = results;
-        kp.stats = stats;
-        kp.arena = E.arena.as<uint64_t>();
-        kp.arena_words = arena_words;
-        kp.mark = E.mark.as<uint64_t>();
-        kp.mark_words = mark_words;
-        kp.cpos = E.cpos.as<uint32_t>();
-        kp.cpos_len = cpos_len;
-        kp.lds_words = tu.lds_words;
-        kp.list_cutoff = tu.list_cutoff;
-        kp.compact_ratio = tu.compact_ratio;
-        kp.sections = (tu.flags & MBE_FLAG_SECTIONS) ? secs : nullptr;
-        HIPC(hipHostGetDevicePointer((void**)&kp.abort, E.abort_flag, 0));
-        kp.node_count = stats + mbe::kStatNodeBudget;
-        kp.node_budget = tu.max_nodes;
-        kp.overflow_count = ovf_count;
-        kp.overflow_list = ovf_list;
-        kp.overflow_cap = ovf_cap;
-        kp.donate_depth = tu.donate_depth;
-        kp.donate_interval = tu.donate_interval;
-        kp.slots = E.slots.as<uint64_t>();
-        kp.slot_words = slot_words;
-        kp.n_slots = n_slots;
-        kp.dq = E.dq.as<unsigned long long>();
-        // all workers start: those without a root task steal donated subtrees
-        uint32_t nw = workers;
-        if (E.progress_slots < 2 * (size_t)nw) {
-            if (E.progress) HIPC(hipHostFree(E.progress));
-            E.progress = nullptr;
-            HIPC(hipHostMalloc((void**)&E.progress, 2 * (size_t)nw * 8, hipHostMallocMapped | hipHostMallocCoherent));
-            E.progress_slots = 2 * (size_t)nw;
-        }
-        memset(E.progress, 0, 2 * (size_t)nw * 8);
-        unsigned long long* dprog = nullptr;
-        HIPC(hipHostGetDevicePointer((void**)&dprog, E.progress, 0));
-        kp.progress = dprog;
-        HIPC(hipEventRecord(E.ev0, E.stream));
-        HIPC(mbe::launch_mbe(kp, nw, E.stream));
         HIPC(hipEventRecord(E.ev1, E.stream));
-        HIPC(wait_with_progress(E, nw, n_tasks, pass, deadline));
+        HIPC(wait_with_progress(E, B, nw, n_tasks, pass, deadline));
         float ms = 0;
         HIPC(hipEventElapsedTime(&ms, E.ev0, E.ev1));
         kernel_ms += ms;
         launches++;
         out.workers = std::max(out.workers, nw);
         unsigned long long novf = 0;
-        HIPC(hipMemcpy(&novf, ovf_count, 8, hipMemcpyDeviceToHost));
+        HIPC(hipMemcpy(&novf, kp.overflow_count, 8, hipMemcpyDeviceToHost));
         if (__atomic_load_n(E.abort_flag, __ATOMIC_SEQ_CST)) break;
         if (!novf) break;
         task_list.resize(novf);
-        HIPC(hipMemcpy(task_list.data(), ovf_list, novf * 8, hipMemcpyDeviceToHost));
+        HIPC(hipMemcpy(task_list.data(), kp.overflow_list, novf * 8, hipMemcpyDeviceToHost));
         n_tasks = novf;
         arena_words *= 16;
         workers = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(workers, budget_words / arena_words));
-        HIPC(hipMemsetAsync(ovf_count, 0, 8, E.stream));
     }
-    std::vector<unsigned long long> st(mbe::kStatCount), rr(2 * ng), sc(mbe::kSecCount);
-    HIPC(hipMemcpy(st.data(), stats, st.size() * 8, hipMemcpyDeviceToHost));
-    HIPC(hipMemcpy(sc.data(), secs, sc.size() * 8, hipMemcpyDeviceToHost));
+    std::vector<unsigned long long> st(mbe::kStatCount), rr(2 * (size_t)m.ng), sc(mbe::kSecCount);
+    HIPC(hipMemcpy(st.data(), kp.stats, st.size() * 8, hipMemcpyDeviceToHost));
+    HIPC(hipMemcpy(sc.data(), kp.stats + mbe::kStatCount, sc.size() * 8, hipMemcpyDeviceToHost));
     for (int i = 0; i < mbe::kSecCount; i++) out.sections[i] = sc[i];
-    HIPC(hipMemcpy(rr.data(), results, rr.size() * 8, hipMemcpyDeviceToHost));
+    HIPC(hipMemcpy(rr.data(), kp.results, rr.size() * 8, hipMemcpyDeviceToHost));
     res.assign(rr.begin(), rr.end());
     out.kernel_ms = kernel_ms;
     out.nodes = st[mbe::kStatNodes];
@@ -358,7 +334,61 @@ mbe_status run_tasks(Engine& E, const std::vector<mbe::Prepared>& ps,
     return MBE_OK;
 }
 
-}  // namespace
+Meta meta_of(const std::vector<mbe::Prepared>& ps, uint64_t total_tasks) {
+    Meta m;
+    m.ng = (uint32_t)ps.size();
+    for (auto& p : ps) m.max_nr = std::max(m.max_nr, p.nR);
+    m.total_tasks = total_tasks;
+    return m;
+}
+
+// Per-graph and total results of a batch from the result words.
+void fill_results(const std::vector<uint64_t>& res, size_t n, mbe_result& agg,
+                  mbe_result* per_graph) {
+    for (size_t i = 0; i < n; i++) {
+        agg.count += res[2 * i];
+        agg.hash += res[2 * i + 1];
+    }
+    if (per_graph)
+        for (size_t i = 0; i < n; i++) {
+            per_graph[i] = agg;
+            per_graph[i].count = res[2 * i];
+            per_graph[i].hash = res[2 * i + 1];
+        }
+}
+
+mbe_status ensure_lds(uint32_t lds_words) {
+    static std::mutex mu;
+    static uint32_t set = 0;
+    std::lock_guard<std::mutex> lock(mu);
+    if (set >= lds_words) return MBE_OK;
+    HIPC(mbe::set_lds_limit(lds_words));
+    set = lds_words;
+    return MBE_OK;
+}
+
+}  // namespace host
+}  // namespace mbe
+
+using namespace mbe::host;
+
+// A batch kept resident on the device: graphs uploaded once, and the first
+// pass (control clears, deque reset, persistent launch) captured as a graph
+// that every run replays.
+struct mbe_batch_plan {
+    std::vector<mbe::Prepared> ps;
+    Meta m;
+    int variant = 0;
+    Tune tu{};
+    Bufs b;
+    hipGraph_t graph = nullptr;
+    Pass0 pass0;
+    ~mbe_batch_plan() {
+        if (pass0.exec) (void)hipGraphExecDestroy(pass0.exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        b.release();
+    }
+};
 
 extern "C" {
 
@@ -480,26 +510,13 @@ mbe_status mbe_batch_solve(const mbe_graph* const* graphs, size_t n, mbe_variant
     mbe_result agg{};
     std::vector<uint64_t> res;
     if (n) {
-        s = upload(E, ps, dg, total_tasks);
+        s = upload(E.b, E.stream, ps, dg, total_tasks);
         if (s != MBE_OK) return s;
-        s = run_tasks(E, ps, dg, total_tasks, (int)variant, tu, res, agg);
+        s = run_tasks(E, E.b, meta_of(ps, total_tasks), (int)variant, tu, res, agg);
         if (s != MBE_OK && s != MBE_ETIMEOUT) return s;  // a timeout keeps partial results
     }
     agg.total_ms = now_ms() - t0;
-    for (size_t i = 0; i < n; i++) {
-        agg.count += res[2 * i];
-        agg.hash += res[2 * i + 1];
-        if (per_graph) {
-            per_graph[i] = agg;
-            per_graph[i].count = res[2 * i];
-            per_graph[i].hash = res[2 * i + 1];
-        }
-    }
-    if (per_graph)
-        for (size_t i = 0; i < n; i++) {
-            per_graph[i].kernel_ms = agg.kernel_ms;
-            per_graph[i].total_ms = agg.total_ms;
-        }
+    fill_results(res, n, agg, per_graph);
     if (total) *total = agg;
     return s;
 }
@@ -512,6 +529,109 @@ mbe_status mbe_solve(const mbe_graph* g, mbe_variant variant, const mbe_tuning* 
     mbe_status s = mbe_batch_solve(one, 1, variant, tuning, nullptr, &r);
     if (s == MBE_OK || s == MBE_ETIMEOUT) *out = r;
     return s;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Capture the plan's first pass into a graph (again after an overflow pass
+// reallocated buffers the previous capture points into).
+mbe_status capture_pass0(Engine& E, mbe_batch_plan* p) {
+    if (p->pass0.exec) (void)hipGraphExecDestroy(p->pass0.exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    p->pass0.exec = nullptr;
+    p->graph = nullptr;
+    mbe_status ls = ensure_lds(p->tu.lds_words);
+    if (ls != MBE_OK) return ls;
+    std::vector<uint64_t> task_list;
+    uint64_t n_tasks = p->m.total_tasks;
+    if (p->m.ng == 1 && (p->tu.root_begin || p->tu.root_end || p->tu.root_stride > 1)) {
+        const uint64_t end = p->tu.root_end ? std::min<uint64_t>(p->tu.root_end, n_tasks) : n_tasks;
+        for (uint64_t r = p->tu.root_begin; r < end; r += p->tu.root_stride) task_list.push_back(r);
+        n_tasks = task_list.size();
+    }
+    mbe_status s = setup_pass(p->b, E.stream, E.abort_flag, p->m, p->tu, p->variant, task_list,
+                              n_tasks, p->tu.arena_words, p->tu.workers, p->pass0.kp);
+    if (s != MBE_OK) return s;
+    p->pass0.workers = p->tu.workers;
+    HIPC(hipStreamSynchronize(E.stream));
+    HIPC(hipStreamBeginCapture(E.stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = enqueue_pass(p->pass0.kp, p->pass0.workers, p->m.ctl_words(), E.stream);
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(E.stream, &g);
+    HIPC(e);
+    HIPC(e2);
+    p->graph = g;
+    HIPC(hipGraphInstantiate(&p->pass0.exec, p->graph, nullptr, nullptr, 0));
+    return MBE_OK;
+}
+
+bool capture_current(const mbe_batch_plan* p) {
+    const mbe::KParams& kp = p->pass0.kp;
+    return p->pass0.exec && kp.arena == p->b.arena.as<uint64_t>() &&
+           kp.mark == p->b.mark.as<uint64_t>() && kp.cpos == p->b.cpos.as<uint32_t>() &&
+           kp.cursor == p->b.ctl.as<unsigned long long>() &&
+           kp.overflow_list == p->b.ovf.as<uint64_t>() && kp.slots == p->b.slots.as<uint64_t>() &&
+           kp.dq == p->b.dq.as<unsigned long long>();
+}
+
+}  // namespace
+
+extern "C" {
+
+mbe_status mbe_batch_plan_create(const mbe_graph* const* graphs, size_t n, mbe_variant variant,
+                                 const mbe_tuning* tuning, mbe_batch_plan** out) {
+    if (!out || !graphs || !n || (unsigned)variant > 3u) return fail(MBE_EINVAL, "bad argument");
+    for (size_t i = 0; i < n; i++)
+        if (!graphs[i]) return fail(MBE_EINVAL, "null graph in batch");
+    Engine& E = engine();
+    std::lock_guard<std::mutex> lock(E.mu);
+    mbe_status s = E.ensure_init();
+    if (s != MBE_OK) return s;
+    auto p = std::make_unique<mbe_batch_plan>();
+    p->tu = resolve(tuning, E.cus, E.lds_per_cu);
+    if (p->tu.side < -1 || p->tu.side > 1) return fail(MBE_EINVAL, "bad side");
+    p->variant = (int)variant;
+    p->ps.resize(n);
+    for (size_t i = 0; i < n; i++) mbe::prepare(graphs[i]->g, p->tu.side, p->ps[i]);
+    std::vector<mbe::DevGraph> dg;
+    uint64_t total_tasks = 0;
+    s = upload(p->b, E.stream, p->ps, dg, total_tasks);
+    if (s != MBE_OK) return s;
+    p->m = meta_of(p->ps, total_tasks);
+    s = capture_pass0(E, p.get());
+    if (s != MBE_OK) return s;
+    *out = p.release();
+    return MBE_OK;
+}
+
+mbe_status mbe_batch_plan_run(mbe_batch_plan* p, mbe_result* per_graph, mbe_result* total) {
+    if (!p) return fail(MBE_EINVAL, "null plan");
+    const double t0 = now_ms();
+    Engine& E = engine();
+    std::lock_guard<std::mutex> lock(E.mu);
+    mbe_status s = E.ensure_init();
+    if (s != MBE_OK) return s;
+    if (!capture_current(p)) {
+        s = capture_pass0(E, p);
+        if (s != MBE_OK) return s;
+    }
+    mbe_result agg{};
+    std::vector<uint64_t> res;
+    s = run_tasks(E, p->b, p->m, p->variant, p->tu, res, agg, &p->pass0);
+    if (s != MBE_OK && s != MBE_ETIMEOUT) return s;
+    agg.total_ms = now_ms() - t0;
+    fill_results(res, p->m.ng, agg, per_graph);
+    if (total) *total = agg;
+    return s;
+}
+
+void mbe_batch_plan_free(mbe_batch_plan* p) {
+    if (!p) return;
+    Engine& E = engine();
+    std::lock_guard<std::mutex> lock(E.mu);
+    delete p;
 }
 
 }  // extern "C"

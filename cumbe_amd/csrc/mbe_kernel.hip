@@ -1372,6 +1372,14 @@ __global__ __launch_bounds__(64) void mbe_persistent(KParams kp) {
     }
 }
 
+// Fill n words with v (clears control words inside captured graphs, where a
+// memset node is not used).
+__global__ void mbe_fill(unsigned long long* p, uint64_t n, unsigned long long v) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
 // Reset the deque control block: head, tail, idle, pending, slot sequences.
 __global__ void mbe_dq_init(unsigned long long* dq, uint32_t n_slots) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1406,6 +1414,14 @@ hipError_t launch_mbe(const KParams& kp, uint32_t workers, hipStream_t stream) {
 hipError_t launch_dq_init(unsigned long long* dq, uint32_t n_slots, hipStream_t stream) {
     const uint32_t n = n_slots > 4 ? n_slots : 4;
     hipLaunchKernelGGL(mbe_dq_init, dim3((n + 255) / 256), dim3(256), 0, stream, dq, n_slots);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill(unsigned long long* p, uint64_t n, unsigned long long v, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    const uint64_t blocks = (n + 255) / 256;
+    hipLaunchKernelGGL(mbe_fill, dim3((uint32_t)(blocks < 1024 ? blocks : 1024)), dim3(256), 0,
+                       stream, p, n, v);
     return hipGetLastError();
 }
 

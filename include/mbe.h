@@ -104,6 +104,8 @@ typedef struct {
 
 #define MBE_FLAG_SECTIONS 1u
 
+typedef struct mbe_batch_plan mbe_batch_plan;  /* resident batch + captured launch */
+
 const char* mbe_status_str(mbe_status s);
 int mbe_last_hip_error(void);       /* hipError_t of the last MBE_EHIP */
 const char* mbe_last_error(void);   /* human-readable detail, thread-local */
@@ -137,10 +139,23 @@ mbe_status mbe_solve(const mbe_graph* g, mbe_variant variant, const mbe_tuning* 
                      mbe_result* out);
 
 /* Many graphs in one persistent launch sharing one task pool; per_graph[i]
- * receives graph i's count and hash (timing fields are the batch's). */
+ * receives graph i's count and hash (timing fields are the batch's).
+ * No reference counterpart: the reference runs one graph per process. */
 mbe_status mbe_batch_solve(const mbe_graph* const* graphs, size_t n, mbe_variant variant,
                            const mbe_tuning* tuning, mbe_result* per_graph,
                            mbe_result* total);
+
+/* Batch plan: the graphs are prepared and uploaded once into a device arena
+ * owned by the plan, and the first pass (control clears, deque reset, the
+ * persistent launch over every root of every graph) is captured into a HIP
+ * graph that each run replays. Roots whose local problem exceeds a worker's
+ * arena are finished by eager follow-up launches, as in mbe_batch_solve.
+ * Runs of one plan, and plans with solves, are serialised. No reference
+ * counterpart (the reference's per-graph launch is src/main.cu:185-204). */
+mbe_status mbe_batch_plan_create(const mbe_graph* const* graphs, size_t n, mbe_variant variant,
+                                 const mbe_tuning* tuning, mbe_batch_plan** out);
+mbe_status mbe_batch_plan_run(mbe_batch_plan* plan, mbe_result* per_graph, mbe_result* total);
+void mbe_batch_plan_free(mbe_batch_plan* plan);
 
 #ifdef __cplusplus
 }
