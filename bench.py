@@ -13,14 +13,26 @@ same kernel path.
 Reported per JSON line (rank 0, one GPU):
   value       bicliques / kernel seconds (HIP events around the enumeration
               launches on the engine's stream; the graph is already in HBM)
-  roofline    algorithmic bytes / launch time against 8 TB/s HBM: 8 bytes per
-              bit-matrix operand word the visited nodes read plus 8 bytes per
-              adjacency entry read while localising roots (each entry: one
-              4-byte id and one 4-byte position lookup)
+  roofline    algorithmic bytes / launch time against 8 TB/s HBM: 8 bytes
+              (one 4-byte neighbour id + one 4-byte position lookup) per
+              adjacency entry W that the canonical serial algorithm scans on
+              this workload (W from the oracle: tests/golden/configs.json);
+              `traffic` is the HBM bytes per launch rocprofv3 measured
+              (profiles/), and `bitmap_GBps` the engine's own operand words
+              (bit-matrix words its nodes read + adjacency entries of root
+              localisation, 8 bytes each) per second
   cpu_baseline the CPU oracle (the reference algorithm restated serially,
               root-parallel over host threads) on a stride-64 sample of the
               roots of rank >= 1000; the GPU solves the same sample and must
               match its count and hash bit-exactly
+  batch       the batched workload's graphs whose oracle result is known
+              (tests/golden/batch_c3.csv): one captured persistent launch
+              (mbe_batch_plan) against one launch per graph (mbe_solve), both
+              checked against the goldens
+  serving     Poisson arrivals at 200/1000/2000 requests/s through the
+              serving loop (cumbe_amd.serve_bench), 99 % small graphs of the
+              batched distribution and 1 % large graphs; latency percentiles
+              next to the same requests served alone
 """
 import argparse
 import json
@@ -38,8 +50,74 @@ HBM_PEAK_GBS = 8000.0
 WORKLOAD = dict(n_u=94000, n_v=30000, m=293000, gamma_u=2.0, gamma_v=1.8, max_deg=7600, seed=2)
 ROOT_BEGIN = 400
 SAMPLE = dict(begin=1000, end=0, stride=64)
+LARGE = (200000, 100000, 1500000, 2.0, 2.0, 0)  # serving's large class (+ seed)
+LARGE_SEEDS = (11, 12)
 TUNING_KEY = "c1_roots400"
 TUNING = dict(lds_words=4096, waves_per_cu=4, donate_interval=64)
+
+
+def golden(name):
+    return json.load(open(os.path.join(ROOT, "tests", "golden", name)))
+
+
+def c3_rows():
+    rows = []
+    for line in open(os.path.join(ROOT, "tests", "golden", "batch_c3.csv")):
+        if line.startswith("#") or line.startswith("seed"):
+            continue
+        s, nu, nv, ne, cnt, h, w = line.strip().split(",")
+        if cnt != "-":
+            rows.append((int(s), int(cnt), int(h, 16)))
+    return rows
+
+
+def batch_leg(variant, reps=3):
+    """Batched plan vs one launch per graph on the golden-known graphs."""
+    rows = c3_rows()
+    gs = [M.Graph.batch_graph(s) for s, _, _ in rows]
+    plan = M.BatchPlan(gs, variant)
+    plan.run()  # warm-up (first replay)
+    best = None
+    ok = True
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        per, tot = plan.run()
+        wall = time.perf_counter() - t0
+        ok &= all((r.count, r.hash) == (c, h) for (_, c, h), r in zip(rows, per))
+        if best is None or wall < best[0]:
+            best = (wall, tot)
+    plan.close()
+    wall_b, tot = best
+    t0 = time.perf_counter()
+    kms = 0.0
+    for (s, c, h), g in zip(rows, gs):
+        r = M.solve(g, variant)
+        kms += r.kernel_ms
+        ok &= (r.count, r.hash) == (c, h)
+    wall_p = time.perf_counter() - t0
+    n_bic = sum(c for _, c, _ in rows)
+    return {
+        "graphs": len(rows),
+        "bicliques": n_bic,
+        "batched_bicliques_per_s": n_bic / wall_b,
+        "batched_wall_ms": 1000 * wall_b,
+        "batched_kernel_ms": tot.kernel_ms,
+        "per_graph_bicliques_per_s": n_bic / wall_p,
+        "per_graph_wall_ms": 1000 * wall_p,
+        "per_graph_kernel_ms_sum": kms,
+        "speedup": wall_p / wall_b,
+        "parity": ok,
+    }
+
+
+def serving_leg(duration, rates):
+    from cumbe_amd import serve_bench as SB
+    import random
+    seeds = [s for s, _, _ in c3_rows()]
+    pick = sorted(random.Random(1).sample(seeds, 512))
+    small = {s: M.Graph.batch_graph(s) for s in pick}
+    large = {s: M.Graph.chung_lu(*LARGE, s) for s in LARGE_SEEDS}
+    return SB.run(small, large, rates, duration, 0.01)
 
 
 def traffic_from_profiles():
@@ -84,6 +162,11 @@ def main():
     ap.add_argument("--variant", default="cuMBE", choices=list(M.VARIANTS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--root-begin", type=int, default=ROOT_BEGIN)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serve-duration", type=float, default=5.0,
+                    help="seconds of arrivals per rate (60 for the full workload)")
+    ap.add_argument("--serve-rates", default="200,1000,2000")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the engine "
@@ -122,9 +205,12 @@ def main():
     wall_ms = (time.perf_counter() - t0) * 1000.0
 
     value = count / (kernel_ms / 1000.0)
-    alg_bytes = 8.0 * words + 8.0 * build
     launch_ms = kernel_ms / max(launches, 1)
-    achieved = alg_bytes / max(launches, 1) / (launch_ms / 1000.0) / 1e9
+    # algorithmic bytes: 8 per adjacency entry of the canonical algorithm
+    wl = golden("configs.json")["c1"]["bench_workload"]
+    w_alg = wl["work"] if args.root_begin == wl["begin"] else None
+    achieved = (8.0 * w_alg / (kernel_ms / 1000.0) / 1e9) if w_alg else None
+    bitmap_gbps = 8.0 * (words + build) / (kernel_ms / 1000.0) / 1e9
     traffic = traffic_from_profiles()
     line = {
         "metric": "maximal bicliques/s (bit-exact)",
@@ -154,11 +240,20 @@ def main():
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
+            "frac": achieved / HBM_PEAK_GBS if achieved else None,
             "traffic": traffic,
+            "work_entries": w_alg,
+            "work_source": wl["work_source"],
+            "bitmap_GBps": bitmap_gbps,
+            "launch_ms": launch_ms,
         },
         "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(g, base),
     }
+    if not args.no_batch:
+        line["batch"] = batch_leg(args.variant)
+    if not args.no_serving:
+        line["serving"] = serving_leg(args.serve_duration,
+                                      [float(x) for x in args.serve_rates.split(",")])
     print(json.dumps(line))
 
 

@@ -80,6 +80,7 @@ class Result(C.Structure):
 SECTION_NAMES = ["other", "root_build", "select", "lprime", "check", "expand", "report",
                  "compact", "donate", "fetch", "wait"]
 FLAG_SECTIONS = 1
+FLAG_VARIANCE = 2
 
 
 _lib = None
@@ -116,10 +117,19 @@ def lib() -> C.CDLL:
     L.mbe_batch_plan_run.argtypes = [G, P(Result), P(Result)]
     L.mbe_batch_plan_free.argtypes = [G]
     L.mbe_batch_plan_free.restype = None
+    L.mbe_last_worker_busy.argtypes = [P(C.c_uint64), C.c_uint32]
+    L.mbe_last_worker_busy.restype = C.c_uint32
+    L.mbe_server_create.argtypes = [P(ServerConfig), P(G)]
+    L.mbe_server_submit.argtypes = [G, G, C.c_int, P(C.c_uint64)]
+    L.mbe_server_poll.argtypes = [G, C.c_uint64, C.c_int, P(RequestResult)]
+    L.mbe_server_get_stats.argtypes = [G, P(ServerStats)]
+    L.mbe_server_destroy.argtypes = [G]
+    L.mbe_server_destroy.restype = None
     for name in ("mbe_graph_load_bi", "mbe_graph_write_bi", "mbe_graph_from_edge_list",
                  "mbe_graph_from_csr", "mbe_graph_info", "mbe_graph_csr", "mbe_gen_chung_lu",
                  "mbe_gen_er_planted", "mbe_gen_batch_graph", "mbe_solve", "mbe_batch_solve",
-                 "mbe_batch_plan_create", "mbe_batch_plan_run"):
+                 "mbe_batch_plan_create", "mbe_batch_plan_run", "mbe_server_create",
+                 "mbe_server_submit", "mbe_server_poll", "mbe_server_get_stats"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -238,6 +248,14 @@ def solve(graph: Graph, variant: str = "cuMBE", tuning: Optional[dict] = None,
     return r
 
 
+def last_worker_busy() -> List[int]:
+    """Per-worker busy ticks (10 ns) of this thread's last FLAG_VARIANCE solve."""
+    n = lib().mbe_last_worker_busy(None, 0)
+    buf = (C.c_uint64 * max(n, 1))()
+    lib().mbe_last_worker_busy(buf, n)
+    return list(buf)[:n]
+
+
 def batch_solve(graphs: List[Graph], variant: str = "cuMBE", tuning: Optional[dict] = None,
                 allow_partial: bool = False):
     """All graphs in one persistent launch; returns (per-graph results, total).
@@ -283,6 +301,75 @@ class BatchPlan:
         if self._h:
             lib().mbe_batch_plan_free(self._h)
             self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("lanes", C.c_uint32), ("max_batch", C.c_uint32), ("queue_cap", C.c_uint32),
+                ("reserved", C.c_uint32), ("hbm_budget", C.c_uint64),
+                ("lane_data_bytes", C.c_uint64), ("tuning", Tuning)]
+
+
+class RequestResult(C.Structure):
+    _fields_ = [("status", C.c_int), ("lane", C.c_uint32), ("count", C.c_uint64),
+                ("hash", C.c_uint64), ("footprint", C.c_uint64), ("queued_ms", C.c_double),
+                ("latency_ms", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("submitted", "completed", "rejected", "launches",
+                                          "queued", "in_flight", "bytes_in_use", "bicliques")]
+
+
+class Server:
+    """Serving loop (mbe_server_*): submit graphs, poll their results."""
+
+    def __init__(self, lanes: int = 0, max_batch: int = 0, hbm_budget: int = 0,
+                 lane_data_bytes: int = 0, tuning: Optional[dict] = None):
+        cfg = ServerConfig()
+        cfg.lanes, cfg.max_batch = lanes, max_batch
+        cfg.hbm_budget, cfg.lane_data_bytes = hbm_budget, lane_data_bytes
+        cfg.tuning.side = -1
+        for k, v in (tuning or {}).items():
+            setattr(cfg.tuning, k, v)
+        h = C.c_void_p()
+        _check(lib().mbe_server_create(C.byref(cfg), C.byref(h)))
+        self._h = h
+
+    def submit(self, graph: Graph, variant: str = "cuMBE") -> int:
+        t = C.c_uint64()
+        _check(lib().mbe_server_submit(self._h, graph.handle, VARIANTS[variant], C.byref(t)))
+        return t.value
+
+    def poll(self, ticket: int, wait_ms: int = 0) -> Optional[RequestResult]:
+        """The request's result, or None if it has not finished."""
+        r = RequestResult()
+        st = lib().mbe_server_poll(self._h, ticket, wait_ms, C.byref(r))
+        if st == -8:
+            return None
+        _check(st)
+        return r
+
+    def stats(self) -> dict:
+        s = ServerStats()
+        _check(lib().mbe_server_get_stats(self._h, C.byref(s)))
+        return {n: getattr(s, n) for n, _ in ServerStats._fields_}
+
+    def close(self) -> None:
+        if self._h:
+            lib().mbe_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
     def __del__(self):
         try:

@@ -14,13 +14,10 @@
 
 #include "host.h"
 
-struct mbe_graph {
-    mbe::Graph g;
-};
-
 namespace {
 thread_local std::string g_err;
 thread_local int g_hip_err = 0;
+thread_local std::vector<uint64_t> g_busy;  // per-worker busy ticks, last variance solve
 }  // namespace
 
 namespace mbe {
@@ -104,16 +101,11 @@ Tune resolve(const mbe_tuning* t, int cus, size_t lds_per_cu) {
 }
 
 // Upload prepared graphs into one device buffer and build the descriptors.
-mbe_status upload(Bufs& B, hipStream_t stream, const std::vector<mbe::Prepared>& ps,
+mbe_status upload(Bufs& B, hipStream_t stream, const std::vector<const mbe::Prepared*>& ps,
                   std::vector<mbe::DevGraph>& dg, uint64_t& total_tasks) {
-    // one padding rule for sizing and placement: every array gets its bytes
-    // plus 8 spare, rounded up to 256
-    auto slot = [](size_t n) { return (n + 8 + 255) & ~(size_t)255; };
+    auto slot = upload_slot;
     size_t bytes = 0;
-    for (auto& p : ps) {
-        bytes += slot(p.offR.size() * 4) + slot(p.adjR.size() * 4) + slot(p.offL.size() * 4) +
-                 slot(p.adjL.size() * 4) + slot(p.hR.size() * 8) + slot(p.hL.size() * 8);
-    }
+    for (const mbe::Prepared* pp : ps) bytes += packed_bytes(*pp);
     HIPC(B.data.ensure(bytes));
     std::vector<char> host(bytes);
     size_t o = 0;
@@ -127,7 +119,7 @@ mbe_status upload(Bufs& B, hipStream_t stream, const std::vector<mbe::Prepared>&
         return (const void*)(dbase + at);
     };
     for (size_t i = 0; i < ps.size(); i++) {
-        auto& p = ps[i];
+        const mbe::Prepared& p = *ps[i];
         auto& d = dg[i];
         d.offR = (const uint32_t*)put(p.offR.data(), p.offR.size() * 4);
         d.adjR = (const uint32_t*)put(p.adjR.data(), p.adjR.size() * 4);
@@ -213,7 +205,12 @@ mbe_status setup_pass(Bufs& B, hipStream_t stream, unsigned long long* abort_hos
     kp.cursor = ctl;
     kp.overflow_count = ctl + 1;
     kp.stats = ctl + 2;
-    kp.sections = (tu.flags & MBE_FLAG_SECTIONS) ? ctl + 2 + mbe::kStatCount : nullptr;
+    const bool variance = tu.flags & MBE_FLAG_VARIANCE;
+    kp.sections = (tu.flags & MBE_FLAG_SECTIONS) || variance ? ctl + 2 + mbe::kStatCount : nullptr;
+    if (variance) {
+        HIPC(B.busy.ensure((size_t)workers * 8));
+        kp.worker_busy = B.busy.as<unsigned long long>();
+    }
     kp.results = ctl + 2 + mbe::kStatCount + mbe::kSecCount;
     kp.arena = B.arena.as<uint64_t>();
     kp.arena_words = arena_words;
@@ -305,6 +302,10 @@ mbe_status run_tasks(Engine& E, Bufs& B, const Meta& m, int variant, const Tune&
         HIPC(hipEventElapsedTime(&ms, E.ev0, E.ev1));
         kernel_ms += ms;
         launches++;
+        if (kp.worker_busy) {
+            g_busy.resize(nw);
+            HIPC(hipMemcpy(g_busy.data(), kp.worker_busy, (size_t)nw * 8, hipMemcpyDeviceToHost));
+        }
         out.workers = std::max(out.workers, nw);
         unsigned long long novf = 0;
         HIPC(hipMemcpy(&novf, kp.overflow_count, 8, hipMemcpyDeviceToHost));
@@ -315,6 +316,10 @@ mbe_status run_tasks(Engine& E, Bufs& B, const Meta& m, int variant, const Tune&
         n_tasks = novf;
         arena_words *= 16;
         workers = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(workers, budget_words / arena_words));
+    }
+    if (!launches) {  // no root task at all: every graph has count 0
+        res.assign(2 * (size_t)m.ng, 0);
+        return MBE_OK;
     }
     std::vector<unsigned long long> st(mbe::kStatCount), rr(2 * (size_t)m.ng), sc(mbe::kSecCount);
     HIPC(hipMemcpy(st.data(), kp.stats, st.size() * 8, hipMemcpyDeviceToHost));
@@ -334,10 +339,16 @@ mbe_status run_tasks(Engine& E, Bufs& B, const Meta& m, int variant, const Tune&
     return MBE_OK;
 }
 
-Meta meta_of(const std::vector<mbe::Prepared>& ps, uint64_t total_tasks) {
+size_t packed_bytes(const mbe::Prepared& p) {
+    return upload_slot(p.offR.size() * 4) + upload_slot(p.adjR.size() * 4) +
+           upload_slot(p.offL.size() * 4) + upload_slot(p.adjL.size() * 4) +
+           upload_slot(p.hR.size() * 8) + upload_slot(p.hL.size() * 8);
+}
+
+Meta meta_of(const std::vector<const mbe::Prepared*>& ps, uint64_t total_tasks) {
     Meta m;
     m.ng = (uint32_t)ps.size();
-    for (auto& p : ps) m.max_nr = std::max(m.max_nr, p.nR);
+    for (const mbe::Prepared* p : ps) m.max_nr = std::max(m.max_nr, p->nR);
     m.total_tasks = total_tasks;
     return m;
 }
@@ -408,6 +419,12 @@ const char* mbe_status_str(mbe_status s) {
 }
 
 int mbe_last_hip_error(void) { return g_hip_err; }
+
+uint32_t mbe_last_worker_busy(uint64_t* out, uint32_t cap) {
+    const size_t n = std::min<size_t>(cap, g_busy.size());
+    if (out && n) memcpy(out, g_busy.data(), n * 8);
+    return (uint32_t)g_busy.size();
+}
 const char* mbe_last_error(void) { return g_err.c_str(); }
 
 mbe_status mbe_graph_load_bi(const char* path, mbe_graph** out) {
@@ -504,15 +521,19 @@ mbe_status mbe_batch_solve(const mbe_graph* const* graphs, size_t n, mbe_variant
     Tune tu = resolve(tuning, E.cus, E.lds_per_cu);
     if (tu.side < -1 || tu.side > 1) return fail(MBE_EINVAL, "bad side");
     std::vector<mbe::Prepared> ps(n);
-    for (size_t i = 0; i < n; i++) mbe::prepare(graphs[i]->g, tu.side, ps[i]);
+    std::vector<const mbe::Prepared*> pp(n);
+    for (size_t i = 0; i < n; i++) {
+        mbe::prepare(graphs[i]->g, tu.side, ps[i]);
+        pp[i] = &ps[i];
+    }
     std::vector<mbe::DevGraph> dg;
     uint64_t total_tasks = 0;
     mbe_result agg{};
     std::vector<uint64_t> res;
     if (n) {
-        s = upload(E.b, E.stream, ps, dg, total_tasks);
+        s = upload(E.b, E.stream, pp, dg, total_tasks);
         if (s != MBE_OK) return s;
-        s = run_tasks(E, E.b, meta_of(ps, total_tasks), (int)variant, tu, res, agg);
+        s = run_tasks(E, E.b, meta_of(pp, total_tasks), (int)variant, tu, res, agg);
         if (s != MBE_OK && s != MBE_ETIMEOUT) return s;  // a timeout keeps partial results
     }
     agg.total_ms = now_ms() - t0;
@@ -594,12 +615,16 @@ mbe_status mbe_batch_plan_create(const mbe_graph* const* graphs, size_t n, mbe_v
     if (p->tu.side < -1 || p->tu.side > 1) return fail(MBE_EINVAL, "bad side");
     p->variant = (int)variant;
     p->ps.resize(n);
-    for (size_t i = 0; i < n; i++) mbe::prepare(graphs[i]->g, p->tu.side, p->ps[i]);
+    std::vector<const mbe::Prepared*> pp(n);
+    for (size_t i = 0; i < n; i++) {
+        mbe::prepare(graphs[i]->g, p->tu.side, p->ps[i]);
+        pp[i] = &p->ps[i];
+    }
     std::vector<mbe::DevGraph> dg;
     uint64_t total_tasks = 0;
-    s = upload(p->b, E.stream, p->ps, dg, total_tasks);
+    s = upload(p->b, E.stream, pp, dg, total_tasks);
     if (s != MBE_OK) return s;
-    p->m = meta_of(p->ps, total_tasks);
+    p->m = meta_of(pp, total_tasks);
     s = capture_pass0(E, p.get());
     if (s != MBE_OK) return s;
     *out = p.release();

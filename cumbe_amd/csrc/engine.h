@@ -95,6 +95,13 @@ struct KParams {
     unsigned long long* abort;
     unsigned long long* node_count;
     uint64_t node_budget;           // 0 = unlimited
+    // per-graph completion (nullable): 2 words per graph in device memory
+    // (open tasks, error bits) and a host-mapped record of 4 words per graph
+    // (count, hash, done flag, error bits)
+    unsigned long long* remaining;
+    unsigned long long* done;
+    // per-worker busy ticks at exit (nullable; needs sections)
+    unsigned long long* worker_busy;
 };
 
 }  // namespace mbe

@@ -11,6 +11,11 @@
 #include "engine.h"
 #include "graph.h"
 
+// A graph handle of the C ABI: the canonical host CSR.
+struct mbe_graph {
+    mbe::Graph g;
+};
+
 namespace mbe {
 hipError_t launch_mbe(const KParams& kp, uint32_t workers, hipStream_t stream);
 hipError_t launch_dq_init(unsigned long long* dq, uint32_t n_slots, hipStream_t stream);
@@ -53,7 +58,7 @@ struct DevBuf {
 // Device buffers of one launch context: the engine's (reused by every solve)
 // or a batch plan's (owned by the plan, whose captured graph points into them).
 struct Bufs {
-    DevBuf arena, mark, cpos, ctl, graphs, data, ovf, slots, dq;
+    DevBuf arena, mark, cpos, ctl, graphs, data, ovf, slots, dq, busy;
     unsigned long long* progress = nullptr;  // host-mapped, 2 per worker
     size_t progress_slots = 0;
     hipError_t ensure_progress(size_t workers) {
@@ -67,7 +72,7 @@ struct Bufs {
         return e;
     }
     void release() {
-        for (DevBuf* b : {&arena, &mark, &cpos, &ctl, &graphs, &data, &ovf, &slots, &dq}) {
+        for (DevBuf* b : {&arena, &mark, &cpos, &ctl, &graphs, &data, &ovf, &slots, &dq, &busy}) {
             if (b->p) (void)hipFree(b->p);
             b->p = nullptr;
             b->bytes = 0;
@@ -103,10 +108,18 @@ constexpr uint32_t kSlots = 1024;  // deque slots
 
 Tune resolve(const mbe_tuning* t, int cus, size_t lds_per_cu);
 
+// Device bytes of one array in a packed upload: its bytes plus 8 spare,
+// rounded up to 256 (one rule for sizing and placement).
+inline size_t upload_slot(size_t n) { return (n + 8 + 255) & ~(size_t)255; }
+// Device bytes of one prepared graph in a packed upload.
+size_t packed_bytes(const Prepared& p);
+
 // Pack prepared graphs into B.data and their descriptors into B.graphs
 // (copies on `stream`, synchronised before returning).
-mbe_status upload(Bufs& B, hipStream_t stream, const std::vector<Prepared>& ps,
+mbe_status upload(Bufs& B, hipStream_t stream, const std::vector<const Prepared*>& ps,
                   std::vector<DevGraph>& dg, uint64_t& total_tasks);
+
+Meta meta_of(const std::vector<const Prepared*>& ps, uint64_t total_tasks);
 
 // Size the buffers of one pass and fill its kernel parameters; abort_host is
 // the host-mapped watchdog flag the kernel polls.

@@ -17,6 +17,7 @@
 #include <iostream>
 #include <sstream>
 #include <string>
+#include <vector>
 
 #include "mbe.h"
 
@@ -89,8 +90,20 @@ int main(int argc, char** argv) {
     mbe_graph_info(g, &nu, &nv, &ne);
     uint32_t nr = nu <= nv ? nu : nv, nl = nu <= nv ? nv : nu;  // src/main.cu:65-70
 
+    // MBE_SECTIONS=1 adds the per-section time split, like the reference's
+    // DESECTION build (src/header.cuh:25-32, src/main.cu:208-216)
+    const char* sec_env = getenv("MBE_SECTIONS");
+    const bool sections = sec_env && atoi(sec_env) != 0;
+    // MBE_VARIANCE=1 prints each worker's busy ticks between the header and
+    // the result lines, like the reference's DEVARIANCE build
+    // (src/mbe_noES.cuh:49-55, read by utils/plot_variance.py)
+    const char* var_env = getenv("MBE_VARIANCE");
+    const bool variance = var_env && atoi(var_env) != 0;
+    mbe_tuning tun{};
+    tun.side = -1;
+    tun.flags = (sections ? MBE_FLAG_SECTIONS : 0) | (variance ? MBE_FLAG_VARIANCE : 0);
     mbe_result r{};
-    s = mbe_solve(g, var, nullptr, &r);
+    s = mbe_solve(g, var, &tun, &r);
     mbe_graph_free(g);
     if (s != MBE_OK) return die(s, "solve");
 
@@ -103,9 +116,24 @@ int main(int argc, char** argv) {
     os << "dataset: " << dataset << "\n";
     os << "|R|: " << nr << ", |L|: " << nl << ", |E|: " << ne << "\n";
     os << "grid_size: " << r.workers << ", block_size: " << 64 << "\n";
+    if (variance) {
+        std::vector<uint64_t> busy(mbe_last_worker_busy(nullptr, 0));
+        mbe_last_worker_busy(busy.data(), (uint32_t)busy.size());
+        for (uint64_t b : busy) os << b << "\n";
+    }
     os << "maximal bicliques: " << r.count << "\n";
-    os << "hash: 0x" << std::hex << std::setw(16) << std::setfill('0') << r.hash << std::dec << "\n";
+    if (sections) {
+        double sum = 0;
+        for (int i = 0; i < 11; i++) sum += (double)r.sections[i];
+        os << "time percentage:" << std::fixed << std::setprecision(4);
+        for (int i = 0; i < 11; i++) os << " " << (sum > 0 ? r.sections[i] * 100.0 / sum : 0.0);
+        os << std::setprecision(6) << "\n";
+        os.unsetf(std::ios::floatfield);
+    }
     os << "runtime (s): " << r.kernel_ms / 1000.0 << "\n";
+    if (!variance)  // the variance parser takes the last two lines as results
+        os << "hash: 0x" << std::hex << std::setw(16) << std::setfill('0') << r.hash << std::dec
+           << "\n";
     std::cout << os.str();
     fout << os.str();
     return 0;

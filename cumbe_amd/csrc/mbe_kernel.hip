@@ -933,10 +933,16 @@ __device__ bool try_donate(const KParams& kp, const Local& S, uint32_t cur, uint
             const uint32_t t = keep ? split_point(T, P, keep) : 0;
             const SplitSrc src{L, P, Q, X, t, running ? (uint32_t)F[0] : 0xFFFFFFFFu};
             if (snapshot_words(kp, T, src) == ~(size_t)0) continue;
-            if (lane() == 0) atomicAdd(kp.dq + 3, 1ull);  // pending, before publishing
+            if (lane() == 0) {  // pending and the graph's open tasks, before publishing
+                atomicAdd(kp.dq + 3, 1ull);
+                if (kp.remaining) atomicAdd(kp.remaining + 2 * (size_t)g, 1ull);
+            }
             int64_t pos = dq_claim_push(kp);
             if (pos < 0) {
-                if (lane() == 0) atomicAdd(kp.dq + 3, ~0ull);
+                if (lane() == 0) {
+                    atomicAdd(kp.dq + 3, ~0ull);
+                    if (kp.remaining) atomicAdd(kp.remaining + 2 * (size_t)g, ~0ull);
+                }
                 return false;
             }
             u64* slot = (u64*)kp.slots + (size_t)((u64)pos % kp.n_slots) * kp.slot_words;
@@ -1235,6 +1241,30 @@ __device__ int build_root(const KParams& kp, const DevGraph& G, uint32_t x, u64*
     return 1;
 }
 
+// Per-graph completion (serving): one task of graph g -- a root or a stolen
+// subtree -- has finished after adding its results. The wave that closes the
+// graph's last open task copies the graph's count, hash and error bits to
+// its host-mapped completion record and then raises the record's flag.
+// Donations add open tasks before they are published (try_donate), and the
+// donor's own task stays open meanwhile, so the count cannot reach zero
+// early. Roots deferred for a larger arena stay open until their pass.
+__device__ void finish_task(const KParams& kp, uint32_t g, bool failed) {
+    if (!kp.remaining || lane() != 0) return;
+    unsigned long long* rem = kp.remaining + 2 * (size_t)g;
+    if (failed) atomicOr(rem + 1, 1ull);
+    const u64 left = __hip_atomic_fetch_add(rem, ~0ull, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (left != 1 || !kp.done) return;
+    unsigned long long* d = kp.done + 4 * (size_t)g;
+    const u64 c = __hip_atomic_load(&kp.results[2 * g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const u64 h = __hip_atomic_load(&kp.results[2 * g + 1], __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT);
+    const u64 e = __hip_atomic_load(rem + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(d, c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(d + 1, h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(d + 3, e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(d + 2, 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 template <int V>
 __global__ __launch_bounds__(64) void mbe_persistent(KParams kp) {
     extern __shared__ u64 lds_mem[];
@@ -1289,6 +1319,7 @@ __global__ __launch_bounds__(64) void mbe_persistent(KParams kp) {
                         atomicAdd(&kp.results[2 * g], cnt);
                         atomicAdd(&kp.results[2 * g + 1], hs);
                     }
+                    if (kp.remaining) finish_task(kp, g, aborted(kp));
                 }
                 if (lane() == 0) {
                     atomicAdd(pending, ~0ull);
@@ -1319,6 +1350,7 @@ __global__ __launch_bounds__(64) void mbe_persistent(KParams kp) {
                 sec.mark(kSecFetch);
                 if (stop) {
                     // watchdog: discard the subtree
+                    finish_task(kp, g, true);
                 } else if (ok) {
                     const DevGraph G = kp.graphs[g];
                     u64 cnt = 0, hs = 0;
@@ -1329,8 +1361,10 @@ __global__ __launch_bounds__(64) void mbe_persistent(KParams kp) {
                         atomicAdd(&kp.results[2 * g], cnt);
                         atomicAdd(&kp.results[2 * g + 1], hs);
                     }
+                    if (kp.remaining) finish_task(kp, g, aborted(kp));
                 } else {
                     dropped++;
+                    finish_task(kp, g, true);
                 }
                 if (lane() == 0) {
                     atomicAdd(pending, ~0ull);
@@ -1367,6 +1401,12 @@ __global__ __launch_bounds__(64) void mbe_persistent(KParams kp) {
         atomicAdd(&kp.stats[kStatWords], words);
         if (kp.sections)
             for (int i = 0; i < kSecCount; i++) atomicAdd(&kp.sections[i], sec.t[i]);
+        if (kp.worker_busy) {  // per-worker busy ticks (all sections but the wait)
+            u64 busy = 0;
+            for (int i = 0; i < kSecCount; i++)
+                if (i != kSecWait) busy += sec.t[i];
+            kp.worker_busy[wid] = busy;
+        }
     }
     if (lane() == 0) {
     }

@@ -103,12 +103,20 @@ typedef struct {
 } mbe_result;
 
 #define MBE_FLAG_SECTIONS 1u
+/* Also record each worker's busy time (all sections but the idle wait), the
+ * reference's DEVARIANCE build (src/mbe_noES.cuh:49-55); read it with
+ * mbe_last_worker_busy. Implies MBE_FLAG_SECTIONS. */
+#define MBE_FLAG_VARIANCE 2u
 
 typedef struct mbe_batch_plan mbe_batch_plan;  /* resident batch + captured launch */
 
 const char* mbe_status_str(mbe_status s);
 int mbe_last_hip_error(void);       /* hipError_t of the last MBE_EHIP */
 const char* mbe_last_error(void);   /* human-readable detail, thread-local */
+/* Per-worker busy ticks (10 ns) of the last launch of the last solve on
+ * this thread that ran with MBE_FLAG_VARIANCE: copies up to cap values into
+ * out and returns the number of workers. */
+uint32_t mbe_last_worker_busy(uint64_t* out, uint32_t cap);
 
 /* ------------------------------------------------------------- graphs -- */
 mbe_status mbe_graph_load_bi(const char* path, mbe_graph** out);
@@ -156,6 +164,66 @@ mbe_status mbe_batch_plan_create(const mbe_graph* const* graphs, size_t n, mbe_v
                                  const mbe_tuning* tuning, mbe_batch_plan** out);
 mbe_status mbe_batch_plan_run(mbe_batch_plan* plan, mbe_result* per_graph, mbe_result* total);
 void mbe_batch_plan_free(mbe_batch_plan* plan);
+
+/* ------------------------------------------------------------ serving -- */
+/* A request loop with continuous batching. No reference counterpart (the
+ * reference solves one graph per process, src/main.cu:26-234).
+ *
+ * Submitted graphs are prepared on the caller's thread and queued. A
+ * scheduler thread admits queued requests in arrival order while the sum of
+ * admitted footprints stays within hbm_budget -- footprint = packed CSR bytes
+ * plus worst-case stack (the largest local problem any root can build, in
+ * bytes, times the workers that can hold one of its roots at once) -- and
+ * starts them on the first free lane: a stream with its own resident buffers
+ * and one persistent launch over all roots of the requests it took. Lanes
+ * run concurrently on the device, so requests that arrive while others run
+ * start as soon as a lane and budget are free, and their work overlaps on
+ * the GPU. Each request completes as soon as its own last task does (the
+ * kernel raises a per-request flag in host-mapped memory), not at the end of
+ * its launch. Results are bit-identical to mbe_solve. The server object is
+ * internally synchronised; submit and poll may be called from any thread. */
+typedef struct mbe_server mbe_server;
+
+typedef struct {
+    uint32_t lanes;          /* concurrent launch lanes; 0 = 4 */
+    uint32_t max_batch;      /* requests per launch; 0 = 256 */
+    uint32_t queue_cap;      /* queued (unadmitted) requests; 0 = 1 << 20 */
+    uint32_t reserved;
+    uint64_t hbm_budget;     /* admission budget in bytes; 0 = 64 GiB */
+    uint64_t lane_data_bytes;/* resident CSR capacity per lane; 0 = 256 MiB */
+    mbe_tuning tuning;       /* launch tuning of every lane (NULL fields = defaults) */
+} mbe_server_config;
+
+typedef struct {
+    mbe_status status;       /* MBE_OK, or the request's failure */
+    uint32_t lane;           /* lane that ran it */
+    uint64_t count;
+    uint64_t hash;
+    uint64_t footprint;      /* admitted bytes */
+    double queued_ms;        /* submit -> launch start */
+    double latency_ms;       /* submit -> completion seen by the scheduler */
+} mbe_request_result;
+
+typedef struct {
+    uint64_t submitted, completed, rejected, launches;
+    uint64_t queued, in_flight, bytes_in_use;
+    uint64_t bicliques;      /* sum of completed counts */
+} mbe_server_stats;
+
+mbe_status mbe_server_create(const mbe_server_config* cfg, mbe_server** out);
+/* Queue a graph (copied; the caller may free it afterwards). Returns
+ * MBE_EOVERBUDGET if its footprint alone exceeds the budget, MBE_EBUSY when
+ * the queue is full. */
+mbe_status mbe_server_submit(mbe_server* s, const mbe_graph* g, mbe_variant variant,
+                             uint64_t* ticket);
+/* Wait up to wait_ms (0 = just check, < 0 = forever) for a request; on
+ * MBE_OK the result is moved out and the ticket is retired. MBE_ENOTREADY
+ * if it has not finished; MBE_EINVAL for an unknown ticket. */
+mbe_status mbe_server_poll(mbe_server* s, uint64_t ticket, int wait_ms, mbe_request_result* out);
+mbe_status mbe_server_get_stats(mbe_server* s, mbe_server_stats* out);
+/* Finish every admitted and queued request, stop the scheduler, release
+ * the device buffers. */
+void mbe_server_destroy(mbe_server* s);
 
 #ifdef __cplusplus
 }

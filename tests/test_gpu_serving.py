@@ -1,0 +1,106 @@
+"""GPU tests of the serving loop (mbe_server_*): every request's count and
+hash is the same alone and in a mix of concurrent requests, equals the
+oracle's golden vector, and survives arena-overflow passes; admission
+rejects what the HBM budget cannot hold.
+
+Goldens: tests/golden/batch_c3.csv (the oracle per seed of the batched
+workload's size distribution).
+"""
+import os
+import sys
+import threading
+import time
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+pytestmark = pytest.mark.gpu
+M = pytest.importorskip("cumbe_amd")
+
+
+def say(msg):
+    print(f"[gpu] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def golden_rows(max_work, n, min_work=0):
+    rows = []
+    for line in open(os.path.join(ROOT, "tests", "golden", "batch_c3.csv")):
+        if line.startswith("#") or line.startswith("seed"):
+            continue
+        s, nu, nv, ne, cnt, h, w = line.strip().split(",")
+        if cnt != "-" and min_work <= int(w) <= max_work:
+            rows.append((int(s), int(cnt), int(h, 16)))
+    return rows[:n]
+
+
+def collect(srv, tickets, timeout_s=120):
+    out = {}
+    t0 = time.time()
+    for k, t in tickets.items():
+        r = srv.poll(t, wait_ms=int(max(1, timeout_s - (time.time() - t0)) * 1000))
+        assert r is not None, f"request {k} did not finish"
+        out[k] = r
+    return out
+
+
+def test_alone_equals_mix_equals_golden():
+    rows = golden_rows(50_000_000, 48) + golden_rows(2_000_000_000, 4, min_work=200_000_000)
+    graphs = {s: M.Graph.batch_graph(s) for s, _, _ in rows}
+    with M.Server(lanes=4, max_batch=16) as srv:
+        alone = {}
+        for s, _, _ in rows:
+            say(f"alone {s}")
+            alone[s] = collect(srv, {s: srv.submit(graphs[s])})[s]
+        # the mix: two submitting threads, interleaved, while earlier
+        # requests are still on the GPU
+        tickets = {}
+        lock = threading.Lock()
+
+        def feed(part):
+            for s, _, _ in part:
+                t = srv.submit(graphs[s])
+                with lock:
+                    tickets[s] = t
+                time.sleep(0.0005)
+
+        th = [threading.Thread(target=feed, args=(rows[i::2],)) for i in range(2)]
+        say("mix")
+        for x in th:
+            x.start()
+        for x in th:
+            x.join()
+        mix = collect(srv, tickets)
+        st = srv.stats()
+    for s, c, h in rows:
+        a, m = alone[s], mix[s]
+        assert a.status == 0 and m.status == 0, s
+        assert (a.count, a.hash) == (c, h), s
+        assert (m.count, m.hash) == (c, h), s
+    assert st["completed"] == 2 * len(rows) and st["in_flight"] == 0
+    assert st["bytes_in_use"] == 0
+
+
+def test_overflow_passes_and_variants():
+    rows = golden_rows(20_000_000, 12, min_work=1_000_000)
+    graphs = [(s, c, h, M.Graph.batch_graph(s)) for s, c, h in rows]
+    with M.Server(lanes=2, max_batch=8, tuning={"arena_words": 1024, "lds_words": 512}) as srv:
+        tickets = [(s, c, h, v, srv.submit(g, v)) for (s, c, h, g) in graphs
+                   for v in ("cuMBE", "noES")]
+        say("overflow + variants")
+        for s, c, h, v, t in tickets:
+            r = srv.poll(t, wait_ms=120_000)
+            assert r is not None and r.status == 0, (s, v)
+            assert (r.count, r.hash) == (c, h), (s, v)
+
+
+def test_admission_budget():
+    small = M.Graph.batch_graph(golden_rows(100_000, 1)[0][0])
+    big = M.Graph.chung_lu(20000, 8000, 200000, 2.0, 2.0, 0, 5)
+    with M.Server(lanes=1, hbm_budget=64 << 20) as srv:
+        with pytest.raises(M.MbeError) as e:
+            srv.submit(big)
+        assert e.value.status == -5
+        r = srv.poll(srv.submit(small), wait_ms=60_000)
+        assert r is not None and r.status == 0 and r.footprint <= 64 << 20
+        assert srv.stats()["rejected"] == 1
