@@ -50,7 +50,9 @@ HBM_PEAK_GBS = 8000.0
 WORKLOAD = dict(n_u=94000, n_v=30000, m=293000, gamma_u=2.0, gamma_v=1.8, max_deg=7600, seed=2)
 ROOT_BEGIN = 400
 SAMPLE = dict(begin=1000, end=0, stride=64)
-LARGE = (200000, 100000, 1500000, 2.0, 2.0, 0)  # serving's large class (+ seed)
+# serving's large class (+ seed): 200,000 x 100,000, 1.5M edges, exponent 2.0,
+# degrees capped at 128 (uncapped, its hub core is not enumerable in minutes)
+LARGE = (200000, 100000, 1500000, 2.0, 2.0, 128)
 LARGE_SEEDS = (11, 12)
 TUNING_KEY = "c1_roots400"
 TUNING = dict(lds_words=4096, waves_per_cu=4, donate_interval=64)

@@ -310,13 +310,13 @@ class BatchPlan:
 
 
 class ServerConfig(C.Structure):
-    _fields_ = [("lanes", C.c_uint32), ("max_batch", C.c_uint32), ("queue_cap", C.c_uint32),
-                ("reserved", C.c_uint32), ("hbm_budget", C.c_uint64),
-                ("lane_data_bytes", C.c_uint64), ("tuning", Tuning)]
+    _fields_ = [("table_cap", C.c_uint32), ("queue_cap", C.c_uint32), ("max_nr", C.c_uint32),
+                ("reserved", C.c_uint32), ("hbm_budget", C.c_uint64), ("data_bytes", C.c_uint64),
+                ("staging_bytes", C.c_uint64), ("tuning", Tuning)]
 
 
 class RequestResult(C.Structure):
-    _fields_ = [("status", C.c_int), ("lane", C.c_uint32), ("count", C.c_uint64),
+    _fields_ = [("status", C.c_int), ("route", C.c_uint32), ("count", C.c_uint64),
                 ("hash", C.c_uint64), ("footprint", C.c_uint64), ("queued_ms", C.c_double),
                 ("latency_ms", C.c_double)]
 
@@ -329,11 +329,11 @@ class ServerStats(C.Structure):
 class Server:
     """Serving loop (mbe_server_*): submit graphs, poll their results."""
 
-    def __init__(self, lanes: int = 0, max_batch: int = 0, hbm_budget: int = 0,
-                 lane_data_bytes: int = 0, tuning: Optional[dict] = None):
+    def __init__(self, hbm_budget: int = 0, table_cap: int = 0, data_bytes: int = 0,
+                 staging_bytes: int = 0, tuning: Optional[dict] = None):
         cfg = ServerConfig()
-        cfg.lanes, cfg.max_batch = lanes, max_batch
-        cfg.hbm_budget, cfg.lane_data_bytes = hbm_budget, lane_data_bytes
+        cfg.hbm_budget, cfg.table_cap = hbm_budget, table_cap
+        cfg.data_bytes, cfg.staging_bytes = data_bytes, staging_bytes
         cfg.tuning.side = -1
         for k, v in (tuning or {}).items():
             setattr(cfg.tuning, k, v)

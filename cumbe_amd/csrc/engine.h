@@ -58,6 +58,30 @@ constexpr uint32_t kLdsHeaderWords = 344;
 // Deque slot header for donated subtrees (see mbe_kernel.hip).
 constexpr uint32_t kSlotHeaderWords = 8;
 
+// Serving mode: requests arrive while the kernel runs. The host writes a
+// request's packed CSR into pinned staging memory and a record into the
+// mailbox, then bumps the published count; an idle wave (the loader) copies
+// the CSR into the device data buffer, appends the graph to the table and
+// its roots to the task space.
+struct MailRec {
+    uint64_t src;        // u64 offset of the packed CSR in staging
+    uint64_t dst;        // u64 offset in the device data buffer
+    uint64_t words;      // packed size in u64 words
+    uint64_t off[6];     // u64 offsets of offR, adjR, offL, adjL, hR, hL in the block
+    uint32_t nR, nL, n_roots, r_is_u;
+};
+
+struct ServeCtl {                      // device memory, zeroed before launch
+    unsigned long long tasks_avail;    // root tasks in the task space
+    unsigned long long graphs_avail;   // graphs in the table
+    unsigned long long taken;          // mailbox records loaded
+    unsigned long long lock;           // loader lock
+    unsigned long long next_poll;      // wall clock of the next mailbox poll
+    unsigned long long pub_seen;       // mirrors of the host's published count,
+    unsigned long long close_seen;     // close flag and abort flag, refreshed by
+    unsigned long long abort_seen;     // the polling loader
+};
+
 struct KParams {
     const DevGraph* graphs;
     uint32_t n_graphs;
@@ -102,6 +126,16 @@ struct KParams {
     unsigned long long* done;
     // per-worker busy ticks at exit (nullable; needs sections)
     unsigned long long* worker_busy;
+    // serving mode (serve != null): graphs is the growing table, results,
+    // remaining and done are indexed by table position
+    ServeCtl* serve;
+    const MailRec* mail;                  // host-mapped ring of mail_cap records
+    uint64_t mail_cap;
+    const unsigned long long* mail_pub;   // host-mapped: records published
+    unsigned long long* mail_taken;       // host-mapped: records loaded
+    const uint64_t* staging;              // host-mapped staging buffer
+    uint64_t* data;                       // device data buffer
+    const unsigned long long* close;      // host-mapped: no more requests
 };
 
 }  // namespace mbe

@@ -66,7 +66,6 @@ constexpr uint32_t kColVecMax = 8;               // Wc up to which columns are
 enum Variant : int { kCuMBE = 0, kNoRS = 1, kNoES = 2, kNoWS = 3 };
 
 __device__ __forceinline__ uint32_t lane() { return threadIdx.x; }
-__device__ __forceinline__ uint32_t uni32(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 __device__ __forceinline__ u64 uni64(u64 v) {
     uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
     uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
@@ -182,8 +181,9 @@ struct Sec {
     }
 };
 
-__device__ __forceinline__ uint32_t find_graph(const KParams& kp, uint64_t t) {
-    uint32_t lo = 0, hi = kp.n_graphs;
+// Graph of global task t among the first n graphs (task_base ascending).
+__device__ __forceinline__ uint32_t find_graph(const KParams& kp, uint32_t n, uint64_t t) {
+    uint32_t lo = 0, hi = n;
     while (hi - lo > 1) {
         uint32_t mid = (lo + hi) >> 1;
         if (kp.graphs[mid].task_base <= t) lo = mid; else hi = mid;
@@ -595,15 +595,19 @@ __device__ NodeOut process_fwd(const Local& S, u64* F, u64* Fc, uint32_t nl) {
 // then hl[d], hc[nc], rows[nc * W], P[Wc], Q[Wc]: a LEVEL(L, R, P, Q) call
 // over a compacted universe (L = all d bits).
 
-__device__ __forceinline__ u64 ld_acq(unsigned long long* p) {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
 __device__ __forceinline__ u64 ld_rlx(unsigned long long* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void st_rel(unsigned long long* p, u64 v) {
     __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ __forceinline__ void st_rlx(unsigned long long* p, u64 v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Acquire-only fence: waits for this wave's memory operations and
+// invalidates its stale cache lines, without writing the L2 back (a release
+// or a full fence would write back every dirty line of the XCD's L2).
+__device__ __forceinline__ void fence_acq() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
 
 // Claim a slot to write (lane 0 decides); returns its position or -1 if full.
 __device__ int64_t dq_claim_push(const KParams& kp) {
@@ -612,7 +616,7 @@ __device__ int64_t dq_claim_push(const KParams& kp) {
         unsigned long long* tail = kp.dq + 1;
         u64 pos = ld_rlx(tail);
         for (int it = 0; it < 64; it++) {
-            u64 seq = ld_acq(kp.dq + 4 + (pos % kp.n_slots));
+            u64 seq = ld_rlx(kp.dq + 4 + (pos % kp.n_slots));
             int64_t dif = (int64_t)(seq - pos);
             if (dif == 0) {
                 u64 old = atomicCAS(tail, pos, pos + 1);
@@ -625,7 +629,9 @@ __device__ int64_t dq_claim_push(const KParams& kp) {
             }
         }
     }
-    return (int64_t)uni64((u64)r);
+    r = (int64_t)uni64((u64)r);
+    if (r >= 0) fence_acq();  // the slot's last reader is done with it
+    return r;
 }
 
 __device__ void dq_publish(const KParams& kp, u64 pos) {
@@ -641,7 +647,7 @@ __device__ int64_t dq_claim_pop(const KParams& kp) {
         unsigned long long* head = kp.dq;
         u64 pos = ld_rlx(head);
         for (int it = 0; it < 64; it++) {
-            u64 seq = ld_acq(kp.dq + 4 + (pos % kp.n_slots));
+            u64 seq = ld_rlx(kp.dq + 4 + (pos % kp.n_slots));
             int64_t dif = (int64_t)(seq - (pos + 1));
             if (dif == 0) {
                 u64 old = atomicCAS(head, pos, pos + 1);
@@ -655,14 +661,19 @@ __device__ int64_t dq_claim_pop(const KParams& kp) {
         }
     }
     r = (int64_t)uni64((u64)r);
-    __threadfence();
+    // an empty probe touches no cache; a claimed slot is read after an
+    // acquire (its writer published it with a release)
+    if (r >= 0) fence_acq();
     return r;
 }
 
+// The reader hands the slot back once its loads are complete (their values
+// are already in the worker's memory); it wrote nothing others need, so no
+// L2 write-back.
 __device__ void dq_release(const KParams& kp, u64 pos) {
-    __threadfence();
+    __builtin_amdgcn_s_waitcnt(0);
     wsync();
-    if (lane() == 0) st_rel(kp.dq + 4 + (pos % kp.n_slots), pos + kp.n_slots);
+    if (lane() == 0) st_rlx(kp.dq + 4 + (pos % kp.n_slots), pos + kp.n_slots);
 }
 
 // Source sets of a compaction: the candidates of P at index >= t stay in P,
@@ -1127,7 +1138,9 @@ __device__ int build_root(const KParams& kp, const DevGraph& G, uint32_t x, u64*
             if (v != x) atomicOr(&mark[v >> 6], 1ull << (v & 63));
         }
     }
-    __threadfence();
+    // this wave's atomics, then its own loads: an acquire-only fence orders
+    // them (nothing here is published to other waves)
+    fence_acq();
     wsync();
     const uint32_t nwords = (G.nR + 63) >> 6;
     uint32_t ncl = 0, kxl = 0;
@@ -1142,7 +1155,7 @@ __device__ int build_root(const KParams& kp, const DevGraph& G, uint32_t x, u64*
     off = off_cand + (S.nc + 1) / 2;
     if (!layout(kp, S, arena, lds, off, kLdsHdr)) {
         for (uint32_t w = lane(); w < nwords; w += 64) mark[w] = 0;
-        __threadfence();
+        fence_acq();
         wsync();
         return -1;
     }
@@ -1170,7 +1183,7 @@ __device__ int build_root(const KParams& kp, const DevGraph& G, uint32_t x, u64*
     }
     const size_t nz = (size_t)S.nc * S.W + (size_t)S.d * S.Wc;
     for (size_t i = lane(); i < nz; i += 64) S.rows[i] = 0;
-    __threadfence();
+    fence_acq();
     wsync();
     // rows and cols of the local bit matrix
     for (uint32_t f0 = 0; f0 < T; f0 += 64) {
@@ -1186,7 +1199,7 @@ __device__ int build_root(const KParams& kp, const DevGraph& G, uint32_t x, u64*
             }
         }
     }
-    __threadfence();
+    fence_acq();
     wsync();
 
     // ---- root node: L' = N(x), P = cands below kx, Q = the rest
@@ -1241,6 +1254,154 @@ __device__ int build_root(const KParams& kp, const DevGraph& G, uint32_t x, u64*
     return 1;
 }
 
+// ---------------------------------------------------------------- serving
+constexpr uint64_t kPollTicks = 200;   // mailbox polls at most every 2 us (100 MHz clock)
+constexpr uint32_t kLoadBatch = 64;    // records one loader turn takes
+
+// Claim the next root task of the growing task space (lane 0 decides).
+// The pending count is raised only while a task is visibly there to take:
+// idle waves poll this every iteration, and a count raised on every poll
+// would almost never read zero, so the exit test would never pass.
+__device__ bool serve_claim(const KParams& kp, u64& t) {
+    u64 got = 0, task = 0;
+    if (lane() == 0) {
+        unsigned long long* pending = kp.dq + 3;
+        for (;;) {
+            const u64 avail = ld_rlx(&kp.serve->tasks_avail);
+            const u64 c = ld_rlx(kp.cursor);
+            if (c >= avail) break;
+            atomicAdd(pending, 1ull);
+            if (atomicCAS(kp.cursor, c, c + 1) == c) {
+                got = 1;
+                task = c;
+                break;
+            }
+            atomicAdd(pending, ~0ull);
+        }
+    }
+    t = uni64(task);
+    got = uni64(got);
+    if (got) fence_acq();  // the loader published the task's graph with a release
+    return got != 0;
+}
+
+__device__ __forceinline__ uint32_t serve_graphs(const KParams& kp) {
+    u64 n = 0;
+    if (lane() == 0) n = ld_rlx(&kp.serve->graphs_avail);
+    return (uint32_t)uni64(n);
+}
+
+// Loader: one wave at a time (lock), at most every kPollTicks, takes newly
+// published mailbox records: copies each packed CSR from pinned staging into
+// the device data buffer, appends its descriptor to the graph table and its
+// roots to the task space, then publishes the new counts (release) and
+// reports the records taken to the host. Returns true if it loaded any.
+__device__ bool serve_load(const KParams& kp) {
+    ServeCtl* sc = kp.serve;
+    u64 pub = 0, taken = 0;
+    if (lane() == 0) {
+        const u64 now = (u64)wall_clock64();
+        if (now >= ld_rlx(&sc->next_poll) && atomicCAS(&sc->lock, 0ull, 1ull) == 0ull) {
+            fence_acq();  // the previous loader's counters
+            st_rlx(&sc->next_poll, now + kPollTicks);
+            // the host's flags, mirrored in device memory for the idle waves:
+            // close first, then the final published count, then abort
+            const u64 cl =
+                __hip_atomic_load(kp.close, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+            pub = __hip_atomic_load(kp.mail_pub, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+            const u64 ab = __hip_atomic_load(kp.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            st_rlx(&sc->pub_seen, pub);
+            if (ab) st_rlx(&sc->abort_seen, 1ull);
+            if (cl) st_rel(&sc->close_seen, 1ull);
+            taken = ld_rlx(&sc->taken);
+            if (pub <= taken) {
+                st_rlx(&sc->lock, 0ull);
+                pub = 0;
+            }
+        }
+    }
+    pub = uni64(pub);
+    if (!pub) return false;
+    taken = uni64(taken);
+    const u64 end = min(pub, taken + kLoadBatch);
+    u64 g = uni64(lane() == 0 ? ld_rlx(&sc->graphs_avail) : 0);
+    u64 tb = uni64(lane() == 0 ? ld_rlx(&sc->tasks_avail) : 0);
+    for (u64 i = taken; i < end; i++, g++) {
+        const MailRec* rec = kp.mail + (i % kp.mail_cap);
+        const u64 src = uni64(rec->src), dst = uni64(rec->dst), nw = uni64(rec->words);
+        const u64* from = (const u64*)kp.staging + src;
+        u64* to = (u64*)kp.data + dst;
+        for (u64 w = lane(); w < nw; w += 256) {  // four host reads in flight per lane
+            const u64 a = from[w];
+            const u64 b = w + 64 < nw ? from[w + 64] : 0;
+            const u64 c = w + 128 < nw ? from[w + 128] : 0;
+            const u64 d = w + 192 < nw ? from[w + 192] : 0;
+            to[w] = a;
+            if (w + 64 < nw) to[w + 64] = b;
+            if (w + 128 < nw) to[w + 128] = c;
+            if (w + 192 < nw) to[w + 192] = d;
+        }
+        if (lane() == 0) {
+            DevGraph G;
+            G.offR = (const uint32_t*)(to + rec->off[0]);
+            G.adjR = (const uint32_t*)(to + rec->off[1]);
+            G.offL = (const uint32_t*)(to + rec->off[2]);
+            G.adjL = (const uint32_t*)(to + rec->off[3]);
+            G.hR = (const uint64_t*)(to + rec->off[4]);
+            G.hL = (const uint64_t*)(to + rec->off[5]);
+            G.nR = rec->nR;
+            G.nL = rec->nL;
+            G.n_roots = rec->n_roots;
+            G.r_is_u = rec->r_is_u;
+            G.task_base = tb;
+            ((DevGraph*)kp.graphs)[g] = G;
+            kp.remaining[2 * g] = G.n_roots;
+            kp.remaining[2 * g + 1] = 0;
+            kp.results[2 * g] = 0;
+            kp.results[2 * g + 1] = 0;
+            if (G.n_roots == 0) {  // nothing to enumerate: complete now
+                unsigned long long* d = kp.done + 4 * g;
+                __hip_atomic_store(d, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                __hip_atomic_store(d + 1, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                __hip_atomic_store(d + 3, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                __hip_atomic_store(d + 2, 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+            }
+            tb += G.n_roots;
+        }
+        tb = uni64(tb);
+    }
+    wsync();
+    if (lane() == 0) {
+        __threadfence();
+        st_rel(&sc->graphs_avail, g);
+        st_rel(&sc->tasks_avail, tb);
+        st_rel(&sc->taken, end);
+        __hip_atomic_store(kp.mail_taken, end, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        st_rel(&sc->lock, 0ull);
+    }
+    wsync();
+    return true;
+}
+
+// An idle serving wave leaves when nothing is pending and either the
+// watchdog fired or the host closed the mailbox and every published record
+// has been loaded and every task claimed.
+__device__ bool serve_exit(const KParams& kp, u64 pend, bool stop) {
+    if (pend != 0) return false;
+    u64 ex = 0;
+    if (lane() == 0) {
+        ServeCtl* sc = kp.serve;
+        if (stop) {
+            ex = 1;
+        } else if (ld_rlx(&sc->close_seen)) {  // device mirror of the host's close
+            fence_acq();
+            ex = ld_rlx(&sc->taken) >= ld_rlx(&sc->pub_seen) &&
+                 ld_rlx(kp.cursor) >= ld_rlx(&sc->tasks_avail);
+        }
+    }
+    return uni64(ex) != 0;
+}
+
 // Per-graph completion (serving): one task of graph g -- a root or a stolen
 // subtree -- has finished after adding its results. The wave that closes the
 // graph's last open task copies the graph's count, hash and error bits to
@@ -1252,7 +1413,12 @@ __device__ void finish_task(const KParams& kp, uint32_t g, bool failed) {
     if (!kp.remaining || lane() != 0) return;
     unsigned long long* rem = kp.remaining + 2 * (size_t)g;
     if (failed) atomicOr(rem + 1, 1ull);
-    const u64 left = __hip_atomic_fetch_add(rem, ~0ull, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    // The task's results were added with device-scope atomics, which are
+    // performed at the memory side in issue order once acknowledged: wait
+    // for them, then decrement (relaxed). The wave that takes the count to
+    // zero reads the sums with device-scope loads. No L2 write-back per task.
+    __builtin_amdgcn_s_waitcnt(0);
+    const u64 left = atomicAdd(rem, ~0ull);
     if (left != 1 || !kp.done) return;
     unsigned long long* d = kp.done + 4 * (size_t)g;
     const u64 c = __hip_atomic_load(&kp.results[2 * g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1282,21 +1448,30 @@ __global__ __launch_bounds__(64) void mbe_persistent(KParams kp) {
     sec.init(kp.sections != nullptr, lds_mem);
     for (;;) {
         wsync();
-        const bool stop = aborted(kp);
+        // served kernels poll the device mirror of the host's abort flag
+        const bool stop =
+            kp.serve ? uni64(lane() == 0 ? ld_rlx(&kp.serve->abort_seen) : 0) != 0 : aborted(kp);
         if (stop) roots_left = false;
         if (roots_left) {
             u64 t = 0;
-            if (lane() == 0) {
-                atomicAdd(pending, 1ull);
-                t = atomicAdd(kp.cursor, 1ull);
-                if (t >= kp.n_tasks) atomicAdd(pending, ~0ull);
+            bool got;
+            if (kp.serve) {
+                got = serve_claim(kp, t);
+            } else {
+                if (lane() == 0) {
+                    atomicAdd(pending, 1ull);
+                    t = atomicAdd(kp.cursor, 1ull);
+                    if (t >= kp.n_tasks) atomicAdd(pending, ~0ull);
+                }
+                t = uni64(t);
+                got = t < kp.n_tasks;
             }
-            t = uni64(t);
-            if (t < kp.n_tasks) {
+            if (got) {
                 if (idle && lane() == 0) atomicAdd(kp.dq + 2, ~0ull);
                 idle = false;
+                backoff = 1;
                 const uint64_t task = kp.task_list ? kp.task_list[t] : t;
-                const uint32_t g = find_graph(kp, task);
+                const uint32_t g = find_graph(kp, kp.serve ? serve_graphs(kp) : kp.n_graphs, task);
                 const DevGraph G = kp.graphs[g];
                 const uint32_t x = (uint32_t)(task - G.task_base);
                 u64 cnt = 0, hs = 0;
@@ -1313,6 +1488,8 @@ __global__ __launch_bounds__(64) void mbe_persistent(KParams kp) {
                         u64 idx = atomicAdd(kp.overflow_count, 1ull);
                         if (idx < kp.overflow_cap) kp.overflow_list[idx] = task;
                     }
+                    // served requests have no later pass: report the failure
+                    if (kp.serve) finish_task(kp, g, true);
                 } else {
                     roots++;
                     if (lane() == 0 && cnt) {
@@ -1332,7 +1509,11 @@ __global__ __launch_bounds__(64) void mbe_persistent(KParams kp) {
                 }
                 continue;
             }
-            roots_left = false;
+            if (!kp.serve) roots_left = false;  // served roots keep arriving
+        }
+        if (kp.serve && !stop && serve_load(kp)) {
+            backoff = 1;
+            continue;
         }
         if (kp.n_slots) {
             const int64_t pos = dq_claim_pop(kp);
@@ -1380,12 +1561,12 @@ __global__ __launch_bounds__(64) void mbe_persistent(KParams kp) {
             idle = true;
         }
         u64 pend = 0;
-        if (lane() == 0) pend = ld_acq(pending);
+        if (lane() == 0) pend = ld_rlx(pending);
         pend = uni64(pend);
-        if (pend == 0) break;
+        if (kp.serve ? serve_exit(kp, pend, stop) : pend == 0) break;
         // exponential backoff keeps idle waves off the deque's cache lines
         for (uint32_t i = 0; i < backoff; i++) __builtin_amdgcn_s_sleep(32);
-        backoff = min(backoff * 2, 32u);
+        backoff = min(backoff * 2, kp.serve ? 64u : 32u);
         sec.mark(kSecWait);
     }
     sec.mark(kSecWait);

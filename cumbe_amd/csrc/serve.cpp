@@ -1,6 +1,11 @@
-// Serving loop of libmbe.so: admission by HBM footprint, continuous batching
-// over concurrent launch lanes, per-request completion from the kernel's
-// host-mapped completion records.
+// Serving loop of libmbe.so: one long-lived persistent kernel per generation
+// takes requests while it runs (continuous batching). The host admits
+// requests against an HBM budget, packs each request's CSR into pinned
+// staging memory, places it in a device data ring and publishes a mailbox
+// record; an idle wave of the running kernel loads it and appends its roots
+// to the shared task space, where they mix with every other request's work
+// and donated subtrees. Each request completes when the kernel raises its
+// host-mapped completion flag.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -22,41 +27,54 @@ using namespace mbe::host;
 struct Req {
     uint64_t id = 0;
     mbe::Prepared prep;
+    std::unique_ptr<mbe_graph> graph;  // kept for requests served by the engine path
     int variant = 0;
     uint64_t footprint = 0;
+    uint64_t words = 0;                // packed CSR size (u64 words)
+    bool oversize = false;
     double t_submit = 0, t_start = 0, t_done = 0;
+    uint64_t slot = 0, data_pos = 0, stage_pos = 0;
     mbe_request_result res{};
     bool done = false;
 };
 using ReqP = std::shared_ptr<Req>;
 
-// One lane: a stream, resident buffers, and the requests of its current
-// launch. Lanes never free buffers while serving (a free would wait for the
-// whole device); they only grow for an arena-overflow pass.
-struct Lane {
-    uint32_t index = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev1 = nullptr;
-    Bufs b;
-    DevBuf rem;                             // 2 words per graph: open tasks, errors
-    unsigned long long* abort_host = nullptr;  // host-mapped
-    unsigned long long* done_host = nullptr;   // host-mapped, 4 words per graph
-    uint32_t done_cap = 0;
-    std::vector<ReqP> reqs;
-    std::vector<uint8_t> closed;
-    std::vector<uint64_t> tasks;            // task list of an overflow pass
-    Meta m;
-    mbe::KParams kp{};
-    int variant = 0, pass = 0;
-    uint64_t arena_words = 0;
-    uint32_t workers = 0;
-    bool busy = false;
+// Circular allocator in u64 words with in-order reclaim: blocks never wrap
+// (a block that would cross the end starts at the next lap), and space comes
+// back when the oldest live block is released.
+struct Ring {
+    uint64_t cap = 0, head = 0, tail = 0;
+    struct Block {
+        uint64_t start, end;
+        bool freed;
+    };
+    std::deque<Block> blocks;
+    bool alloc(uint64_t n, uint64_t& pos) {
+        if (n > cap) return false;
+        uint64_t start = head;
+        if (start % cap + n > cap) start += cap - start % cap;
+        if (start + n - tail > cap) return false;
+        blocks.push_back({start, start + n, false});
+        head = start + n;
+        pos = start;
+        return true;
+    }
+    void release(uint64_t pos) {
+        for (auto& b : blocks)
+            if (b.start == pos && !b.freed) {
+                b.freed = true;
+                break;
+            }
+        while (!blocks.empty() && blocks.front().freed) blocks.pop_front();
+        tail = blocks.empty() ? head : blocks.front().start;
+    }
 };
 
 // Largest arena (u64 words) the local problem of any root of p can take:
-// hash terms, both bit matrices, scratch lists and a full-depth frame stack
-// (the kernel's layout), with the distance-two candidate count bounded by
-// the sum of the root's neighbours' other edges.
+// the degree-prefix and candidate lists of root localisation, then hash
+// terms, both bit matrices, scratch lists and a full-depth frame stack (the
+// kernel's layout), with the distance-two candidate count bounded by the sum
+// of the root's neighbours' other edges.
 uint64_t root_bound_words(const mbe::Prepared& p) {
     std::vector<uint32_t> degL(p.nL);
     for (uint32_t u = 0; u < p.nL; u++) degL[u] = p.offL[u + 1] - p.offL[u];
@@ -70,9 +88,15 @@ uint64_t root_bound_words(const mbe::Prepared& p) {
         const uint64_t data = d + nc + nc * W + d * Wc;
         const uint64_t scratch = (nc + 1) / 2 + 1 + (d + 1) / 2 + 1;
         const uint64_t frames = (std::min(d, nc) + 2) * (4 + W + 3 * Wc);
-        best = std::max(best, data + scratch + frames);
+        const uint64_t lists = (d + 2) / 2 + std::max<uint64_t>(d, (nc + 1) / 2) + 1;
+        best = std::max(best, lists + data + scratch + frames);
     }
     return best;
+}
+
+template <class T>
+hipError_t host_mapped(T*& p, size_t bytes) {
+    return hipHostMalloc((void**)&p, bytes, hipHostMallocMapped | hipHostMallocCoherent);
 }
 
 }  // namespace
@@ -80,24 +104,75 @@ uint64_t root_bound_words(const mbe::Prepared& p) {
 struct mbe_server {
     mbe_server_config cfg{};
     Tune tu{};
-    int cus = 0;
     std::mutex mu;
-    std::condition_variable done_cv;  // poll waiters
-    std::condition_variable work_cv;  // scheduler wake-up
-    std::deque<ReqP> queue;
-    std::map<uint64_t, ReqP> live;    // submitted, not yet collected
-    std::vector<std::unique_ptr<Lane>> lanes;
-    uint64_t next_id = 1;
+    std::condition_variable done_cv, work_cv;
+    std::deque<ReqP> queue;              // admitted later, in arrival order
+    std::map<uint64_t, ReqP> live;       // submitted, not yet collected
+    std::map<uint64_t, ReqP> inflight;   // published to the kernel, by slot
     mbe_server_stats st{};
     std::thread th;
     bool stop = false;
+    bool failed = false;
+    std::string fail_msg;
 
-    void complete(Lane& L, size_t i, mbe_status s, uint64_t count, uint64_t hash) {
-        const ReqP& r = L.reqs[i];
-        L.closed[i] = 1;
+    // device side
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_end = nullptr;
+    Bufs b;
+    DevBuf gtable, rem, sctl, data;
+    mbe::KParams kp{};
+    bool running = false, closing = false;
+    uint64_t seq = 0;                    // records published in this generation
+    // host-mapped
+    mbe::MailRec* mail = nullptr;
+    unsigned long long *mail_pub = nullptr, *mail_taken = nullptr, *close_flag = nullptr;
+    unsigned long long *done = nullptr, *abort_flag = nullptr;
+    uint64_t* staging = nullptr;
+    Ring data_ring, stage_ring;
+    std::deque<std::pair<uint64_t, uint64_t>> staged;  // (record seq, staging pos)
+
+    mbe_status launch_generation() {
+        *mail_pub = 0;
+        *mail_taken = 0;
+        *close_flag = 0;
+        *abort_flag = 0;
+        memset(done, 0, (size_t)cfg.table_cap * 32);
+        Meta m;
+        m.ng = cfg.table_cap;
+        m.max_nr = cfg.max_nr;
+        m.total_tasks = 1u << 20;  // overflow-list capacity
+        mbe_status s = setup_pass(b, stream, abort_flag, m, tu, 0, {}, 0, tu.arena_words,
+                                  tu.workers, kp);
+        if (s != MBE_OK) return s;
+        kp.graphs = gtable.as<mbe::DevGraph>();
+        kp.n_graphs = 0;
+        kp.remaining = rem.as<unsigned long long>();
+        kp.serve = sctl.as<mbe::ServeCtl>();
+        kp.mail_cap = cfg.table_cap;
+        kp.data = data.as<uint64_t>();
+        HIPC(hipHostGetDevicePointer((void**)&kp.done, done, 0));
+        HIPC(hipHostGetDevicePointer((void**)&kp.mail, mail, 0));
+        HIPC(hipHostGetDevicePointer((void**)&kp.mail_pub, mail_pub, 0));
+        HIPC(hipHostGetDevicePointer((void**)&kp.mail_taken, mail_taken, 0));
+        HIPC(hipHostGetDevicePointer((void**)&kp.close, close_flag, 0));
+        HIPC(hipHostGetDevicePointer((void**)&kp.staging, staging, 0));
+        HIPC(mbe::launch_fill((unsigned long long*)sctl.p, sizeof(mbe::ServeCtl) / 8, 0ull, stream));
+        HIPC(enqueue_pass(kp, tu.workers, m.ctl_words(), stream));
+        HIPC(hipEventRecord(ev_end, stream));
+        running = true;
+        closing = false;
+        seq = 0;
+        staged.clear();
+        stage_ring.head = stage_ring.tail = 0;
+        stage_ring.blocks.clear();
+        st.launches++;
+        return MBE_OK;
+    }
+
+    void finish(const ReqP& r, mbe_status s, uint64_t count, uint64_t hash, uint32_t route) {
         r->t_done = now_ms();
         r->res.status = s;
-        r->res.lane = L.index;
+        r->res.route = route;
         r->res.count = count;
         r->res.hash = hash;
         r->res.footprint = r->footprint;
@@ -111,150 +186,162 @@ struct mbe_server {
         done_cv.notify_all();
     }
 
-    // Take the flags of finished requests; when the launch has ended, start
-    // an overflow pass or free the lane. Returns true if anything happened.
-    bool poll_lane(Lane& L) {
+    // Completion flags of published requests; staging space the loader has
+    // copied out. Returns true if anything completed.
+    bool reap() {
         bool any = false;
-        volatile unsigned long long* d = L.done_host;
-        for (size_t i = 0; i < L.reqs.size(); i++) {
-            if (L.closed[i] || !d[4 * i + 2]) continue;
+        volatile unsigned long long* d = done;
+        for (auto it = inflight.begin(); it != inflight.end();) {
+            const uint64_t g = it->first;
+            if (!d[4 * g + 2]) {
+                ++it;
+                continue;
+            }
             __atomic_thread_fence(__ATOMIC_ACQUIRE);
-            complete(L, i, d[4 * i + 3] ? MBE_ENOMEM : MBE_OK, d[4 * i], d[4 * i + 1]);
+            const ReqP r = it->second;
+            finish(r, d[4 * g + 3] ? MBE_ENOMEM : MBE_OK, d[4 * g], d[4 * g + 1], 0);
+            data_ring.release(r->data_pos);
+            it = inflight.erase(it);
             any = true;
         }
-        const hipError_t q = hipEventQuery(L.ev1);
-        if (q == hipErrorNotReady) return any;
-        mbe_status s = MBE_OK;
-        unsigned long long novf = 0;
-        if (q != hipSuccess) {
-            set_hip_error((int)q);
-            s = fail(MBE_EHIP, std::string("serving launch: ") + hipGetErrorString(q));
-        } else if (hipMemcpyAsync(&novf, L.kp.overflow_count, 8, hipMemcpyDeviceToHost,
-                                  L.stream) != hipSuccess ||
-                   hipStreamSynchronize(L.stream) != hipSuccess) {
-            s = MBE_EHIP;
+        const uint64_t taken = __atomic_load_n(mail_taken, __ATOMIC_ACQUIRE);
+        while (!staged.empty() && staged.front().first < taken) {
+            stage_ring.release(staged.front().second);
+            staged.pop_front();
         }
-        if (s == MBE_OK && novf && L.pass < 6) {
-            // roots whose local problem exceeded the arena: rerun them with
-            // 16x arenas (their requests stay open until then)
-            L.tasks.resize(novf);
-            if (hipMemcpyAsync(L.tasks.data(), L.kp.overflow_list, novf * 8,
-                               hipMemcpyDeviceToHost, L.stream) == hipSuccess &&
-                hipStreamSynchronize(L.stream) == hipSuccess) {
-                L.pass++;
-                L.arena_words *= 16;
-                const uint64_t budget_words = (uint64_t)16 << 30 >> 3;  // 16 GiB per lane
-                L.workers = (uint32_t)std::max<uint64_t>(
-                    1, std::min<uint64_t>(L.workers, budget_words / L.arena_words));
-                if (launch(L, L.tasks, 2) == MBE_OK) return true;
-            }
-            s = MBE_ENOMEM;
-        }
-        for (size_t i = 0; i < L.reqs.size(); i++) {
-            if (L.closed[i]) continue;
-            if (d[4 * i + 2]) {
-                __atomic_thread_fence(__ATOMIC_ACQUIRE);
-                complete(L, i, d[4 * i + 3] ? MBE_ENOMEM : MBE_OK, d[4 * i], d[4 * i + 1]);
-            } else {
-                complete(L, i, s == MBE_OK ? MBE_EHIP : s, 0, 0);
-            }
-        }
-        L.reqs.clear();
-        L.closed.clear();
-        L.busy = false;
-        return true;
+        return any;
     }
 
-    // Enqueue one pass of the lane's current requests.
-    mbe_status launch(Lane& L, const std::vector<uint64_t>& tasks, size_t ctl_clear) {
-        mbe_status s = setup_pass(L.b, L.stream, L.abort_host, L.m, tu, L.variant, tasks,
-                                  tasks.empty() ? L.m.total_tasks : tasks.size(), L.arena_words,
-                                  L.workers, L.kp);
-        if (s != MBE_OK) return s;
-        L.kp.remaining = L.rem.as<unsigned long long>();
-        HIPC(hipHostGetDevicePointer((void**)&L.kp.done, L.done_host, 0));
-        HIPC(enqueue_pass(L.kp, L.workers, ctl_clear, L.stream));
-        HIPC(hipEventRecord(L.ev1, L.stream));
-        st.launches++;
-        return MBE_OK;
-    }
-
-    // Admit queued requests (arrival order, one variant per launch) into a
-    // free lane while the budget allows, upload them and launch.
-    bool start_lane(Lane& L) {
-        if (queue.empty()) return false;
-        const int variant = queue.front()->variant;
-        const uint64_t budget = cfg.hbm_budget;
-        std::vector<ReqP> take;
-        uint64_t bytes = 0;
-        while (!queue.empty() && take.size() < cfg.max_batch) {
-            const ReqP& r = queue.front();
-            if (r->variant != variant) break;
-            if (st.bytes_in_use + r->footprint > budget) break;
-            const uint64_t pb = packed_bytes(r->prep);
-            if (!take.empty() && bytes + pb > cfg.lane_data_bytes) break;
-            bytes += pb;
+    // Publish queued requests to the running kernel while the budget, the
+    // rings and the generation's table allow. The lock is released while a
+    // request's CSR is packed into staging.
+    bool publish(std::unique_lock<std::mutex>& lk) {
+        bool any = false;
+        while (!queue.empty() && running && !closing) {
+            const ReqP r = queue.front();
+            if (r->oversize) break;  // runs between generations
+            if (st.bytes_in_use + r->footprint > cfg.hbm_budget) break;
+            if (seq >= cfg.table_cap) break;  // generation full: rotate
+            uint64_t dpos, spos;
+            if (!data_ring.alloc(r->words, dpos)) break;
+            if (!stage_ring.alloc(r->words, spos)) {
+                data_ring.release(dpos);
+                break;
+            }
+            queue.pop_front();
+            st.queued--;
+            st.in_flight++;
             st.bytes_in_use += r->footprint;
-            take.push_back(r);
+            r->slot = seq++;
+            r->data_pos = dpos;
+            r->stage_pos = spos;
+            r->t_start = now_ms();
+            lk.unlock();
+            pack(*r);
+            lk.lock();
+            inflight[r->slot] = r;
+            staged.emplace_back(r->slot, spos);
+            __atomic_store_n(mail_pub, seq, __ATOMIC_RELEASE);
+            any = true;
+        }
+        return any;
+    }
+
+    // Pack the prepared CSR into staging and fill its mailbox record.
+    void pack(const Req& r) {
+        const mbe::Prepared& p = r.prep;
+        uint64_t* base = staging + r.stage_pos % stage_ring.cap;
+        uint64_t o = 0;
+        uint64_t offs[6];
+        auto put = [&](int k, const void* src, size_t bytes) {
+            offs[k] = o;
+            if (bytes) memcpy(base + o, src, bytes);
+            o += upload_slot(bytes) / 8;
+        };
+        put(0, p.offR.data(), p.offR.size() * 4);
+        put(1, p.adjR.data(), p.adjR.size() * 4);
+        put(2, p.offL.data(), p.offL.size() * 4);
+        put(3, p.adjL.data(), p.adjL.size() * 4);
+        put(4, p.hR.data(), p.hR.size() * 8);
+        put(5, p.hL.data(), p.hL.size() * 8);
+        mbe::MailRec& m = mail[r.slot % cfg.table_cap];
+        m.src = r.stage_pos % stage_ring.cap;
+        m.dst = r.data_pos % data_ring.cap;
+        m.words = o;
+        for (int k = 0; k < 6; k++) m.off[k] = offs[k];
+        m.nR = p.nR;
+        m.nL = p.nL;
+        m.n_roots = p.n_roots;
+        m.r_is_u = p.r_is_u ? 1 : 0;
+    }
+
+    void fail_all(mbe_status s) {
+        for (auto& kv : inflight) finish(kv.second, s, 0, 0, 0);
+        inflight.clear();
+        while (!queue.empty()) {
+            st.queued--;
+            st.in_flight++;
+            st.bytes_in_use += queue.front()->footprint;
+            finish(queue.front(), s, 0, 0, 0);
             queue.pop_front();
         }
-        if (take.empty()) return false;
-        st.queued -= take.size();
-        st.in_flight += take.size();
-        const double t = now_ms();
-        std::vector<const mbe::Prepared*> pp;
-        for (auto& r : take) {
-            r->t_start = t;
-            pp.push_back(&r->prep);
-        }
-        L.reqs = take;
-        L.closed.assign(take.size(), 0);
-        L.variant = variant;
-        L.pass = 0;
-        L.arena_words = tu.arena_words;
-        L.workers = tu.workers;
-        L.busy = true;
-        std::vector<mbe::DevGraph> dg;
-        uint64_t total = 0;
-        mbe_status s = upload(L.b, L.stream, pp, dg, total);
-        // open tasks per graph = its roots; graphs without roots finish now
-        std::vector<unsigned long long> rem(2 * take.size(), 0);
-        for (size_t i = 0; i < take.size(); i++) rem[2 * i] = take[i]->prep.n_roots;
-        if (s == MBE_OK && L.rem.ensure(rem.size() * 8) != hipSuccess) s = MBE_ENOMEM;
-        if (s == MBE_OK && hipMemcpyAsync(L.rem.p, rem.data(), rem.size() * 8,
-                                          hipMemcpyHostToDevice, L.stream) != hipSuccess)
-            s = MBE_EHIP;
-        if (s == MBE_OK && hipStreamSynchronize(L.stream) != hipSuccess) s = MBE_EHIP;
-        memset(L.done_host, 0, (size_t)L.done_cap * 32);
-        L.m = meta_of(pp, total);
-        for (size_t i = 0; i < take.size(); i++)
-            if (!take[i]->prep.n_roots) complete(L, i, s, 0, 0);
-        if (s == MBE_OK && total) s = launch(L, {}, L.m.ctl_words());
-        if (s != MBE_OK || !total) {
-            for (size_t i = 0; i < take.size(); i++)
-                if (!L.closed[i]) complete(L, i, s, 0, 0);
-            L.reqs.clear();
-            L.closed.clear();
-            L.busy = false;
-        }
-        return true;
     }
 
     void run() {
         (void)hipSetDevice(0);
         std::unique_lock<std::mutex> lk(mu);
+        double closed_at = 0;
         for (;;) {
             bool any = false;
-            bool busy = false;
-            for (auto& L : lanes)
-                if (L->busy) any |= poll_lane(*L);
-            for (auto& L : lanes)
-                if (!L->busy) any |= start_lane(*L);
-            for (auto& L : lanes) busy |= L->busy;
-            if (stop && queue.empty() && !busy) break;
+            if (running) {
+                any |= reap();
+                any |= publish(lk);
+                // close the generation to finish (stop), to start a new one
+                // (table full) or to run an engine-path request
+                const bool head_big = !queue.empty() && queue.front()->oversize;
+                const bool drain = stop || head_big || (seq >= cfg.table_cap && !queue.empty());
+                if (drain && inflight.empty() && !closing) {
+                    // rotate the generation (table full) or finish (stop)
+                    __atomic_store_n(close_flag, 1ull, __ATOMIC_RELEASE);
+                    closing = true;
+                    closed_at = now_ms();
+                }
+                if (closing) {
+                    const hipError_t q = hipEventQuery(ev_end);
+                    if (q != hipErrorNotReady) {
+                        running = false;
+                        if (q != hipSuccess) {
+                            failed = true;
+                            fail_msg = hipGetErrorString(q);
+                            fail_all(MBE_EHIP);
+                        }
+                        any = true;
+                    } else if (now_ms() - closed_at > 10000.0) {
+                        __atomic_store_n(abort_flag, 1ull, __ATOMIC_RELEASE);
+                    }
+                } else {
+                    const hipError_t q = hipEventQuery(ev_end);
+                    if (q != hipErrorNotReady) {  // the kernel ended by itself: a fault
+                        running = false;
+                        failed = true;
+                        fail_msg = q == hipSuccess ? "serving kernel exited" : hipGetErrorString(q);
+                        fail_all(MBE_EHIP);
+                    }
+                }
+            } else if (!failed && !queue.empty() && queue.front()->oversize) {
+                run_engine(lk);
+                any = true;
+            } else if (!failed && !queue.empty()) {
+                if (launch_generation() != MBE_OK) {
+                    failed = true;
+                    fail_msg = "serving kernel launch failed";
+                    fail_all(MBE_EHIP);
+                }
+                any = true;
+            }
+            if (stop && !running && queue.empty() && inflight.empty()) break;
             if (any) continue;
-            if (busy) {
-                // completions arrive through host-mapped flags: poll them
+            if (running) {
                 lk.unlock();
                 usleep(20);
                 lk.lock();
@@ -263,44 +350,76 @@ struct mbe_server {
             }
         }
     }
+
+    // A request the kernel does not serve (another variant, local problems
+    // beyond the serving workers' arenas, or an enumerated side beyond the
+    // kernel's tables) runs through mbe_solve while no generation is on the
+    // device, so that engine buffers can grow without waiting on the kernel.
+    void run_engine(std::unique_lock<std::mutex>& lk) {
+        ReqP r = queue.front();
+        queue.pop_front();
+        st.queued--;
+        st.in_flight++;
+        st.bytes_in_use += r->footprint;
+        r->t_start = now_ms();
+        lk.unlock();
+        mbe_tuning t{};
+        t.side = tu.side;
+        mbe_result out{};
+        const mbe_status s = mbe_solve(r->graph.get(), (mbe_variant)r->variant, &t, &out);
+        lk.lock();
+        finish(r, s, out.count, out.hash, 1);
+    }
 };
 
 namespace {
 
-void free_lane(Lane& L) {
-    if (L.stream) (void)hipStreamSynchronize(L.stream);
-    L.b.release();
-    if (L.rem.p) (void)hipFree(L.rem.p);
-    L.rem.p = nullptr;
-    if (L.abort_host) (void)hipHostFree(L.abort_host);
-    if (L.done_host) (void)hipHostFree(L.done_host);
-    if (L.ev1) (void)hipEventDestroy(L.ev1);
-    if (L.stream) (void)hipStreamDestroy(L.stream);
-    L = Lane{};
+void free_server(mbe_server* S) {
+    if (S->stream) (void)hipStreamSynchronize(S->stream);
+    S->b.release();
+    for (DevBuf* d : {&S->gtable, &S->rem, &S->sctl, &S->data}) {
+        if (d->p) (void)hipFree(d->p);
+        d->p = nullptr;
+    }
+    for (void* h : {(void*)S->mail, (void*)S->mail_pub, (void*)S->mail_taken, (void*)S->close_flag,
+                    (void*)S->done, (void*)S->abort_flag, (void*)S->staging})
+        if (h) (void)hipHostFree(h);
+    if (S->ev_end) (void)hipEventDestroy(S->ev_end);
+    if (S->stream) (void)hipStreamDestroy(S->stream);
 }
 
-mbe_status make_lane(mbe_server* S, uint32_t index, Lane& L) {
-    L.index = index;
-    HIPC(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking));
-    HIPC(hipEventCreateWithFlags(&L.ev1, hipEventDisableTiming));
-    HIPC(hipHostMalloc((void**)&L.abort_host, 64, hipHostMallocMapped | hipHostMallocCoherent));
-    *L.abort_host = 0;
-    L.done_cap = S->cfg.max_batch;
-    HIPC(hipHostMalloc((void**)&L.done_host, (size_t)L.done_cap * 32,
-                       hipHostMallocMapped | hipHostMallocCoherent));
-    // reserve the resident buffers once, sized for a full launch of
-    // C3-scale graphs; a larger launch grows them
-    HIPC(L.b.data.ensure(S->cfg.lane_data_bytes));
-    Meta m;
-    m.ng = S->cfg.max_batch;
-    m.max_nr = 1 << 13;
-    m.total_tasks = 1 << 20;
-    mbe::KParams kp{};
-    mbe_status s = setup_pass(L.b, L.stream, L.abort_host, m, S->tu, 0, {}, 0, S->tu.arena_words,
-                              S->tu.workers, kp);
+mbe_status make_server(mbe_server* S) {
+    HIPC(hipSetDevice(0));
+    hipDeviceProp_t prop;
+    HIPC(hipGetDeviceProperties(&prop, 0));
+    const size_t lds_per_cu = prop.maxSharedMemoryPerMultiProcessor
+                                  ? prop.maxSharedMemoryPerMultiProcessor
+                                  : 160 * 1024;
+    // serving arenas default to 8 MiB per worker, so that every root of the
+    // batched size distribution fits the kernel
+    if (!S->cfg.tuning.arena_words) S->cfg.tuning.arena_words = 1u << 20;
+    S->tu = resolve(&S->cfg.tuning, prop.multiProcessorCount, lds_per_cu);
+    if (S->tu.side < -1 || S->tu.side > 1) return fail(MBE_EINVAL, "bad side");
+    if (S->tu.root_begin || S->tu.root_end || S->tu.root_stride > 1)
+        return fail(MBE_EINVAL, "root subsets are not served");
+    mbe_status s = ensure_lds(S->tu.lds_words);
     if (s != MBE_OK) return s;
-    HIPC(L.rem.ensure((size_t)2 * S->cfg.max_batch * 8));
-    HIPC(L.b.graphs.ensure(sizeof(mbe::DevGraph) * S->cfg.max_batch));
+    const uint64_t cap = S->cfg.table_cap;
+    HIPC(hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking));
+    HIPC(hipEventCreateWithFlags(&S->ev_end, hipEventDisableTiming));
+    HIPC(S->gtable.ensure(cap * sizeof(mbe::DevGraph)));
+    HIPC(S->rem.ensure(cap * 16));
+    HIPC(S->sctl.ensure(sizeof(mbe::ServeCtl)));
+    HIPC(S->data.ensure(S->cfg.data_bytes));
+    HIPC(host_mapped(S->mail, cap * sizeof(mbe::MailRec)));
+    HIPC(host_mapped(S->mail_pub, 64));
+    HIPC(host_mapped(S->mail_taken, 64));
+    HIPC(host_mapped(S->close_flag, 64));
+    HIPC(host_mapped(S->abort_flag, 64));
+    HIPC(host_mapped(S->done, cap * 32));
+    HIPC(host_mapped(S->staging, S->cfg.staging_bytes));
+    S->data_ring.cap = S->cfg.data_bytes / 8;
+    S->stage_ring.cap = S->cfg.staging_bytes / 8;
     return MBE_OK;
 }
 
@@ -312,31 +431,16 @@ mbe_status mbe_server_create(const mbe_server_config* cfg, mbe_server** out) {
     if (!out) return fail(MBE_EINVAL, "null argument");
     auto S = std::make_unique<mbe_server>();
     if (cfg) S->cfg = *cfg;
-    if (!S->cfg.lanes) S->cfg.lanes = 4;
-    if (!S->cfg.max_batch) S->cfg.max_batch = 256;
+    if (!S->cfg.table_cap) S->cfg.table_cap = 1u << 18;
     if (!S->cfg.queue_cap) S->cfg.queue_cap = 1u << 20;
+    if (!S->cfg.max_nr) S->cfg.max_nr = 1u << 17;
     if (!S->cfg.hbm_budget) S->cfg.hbm_budget = (uint64_t)64 << 30;
-    if (!S->cfg.lane_data_bytes) S->cfg.lane_data_bytes = (uint64_t)256 << 20;
-    if (S->cfg.lanes > 16) return fail(MBE_EINVAL, "at most 16 lanes");
-    HIPC(hipSetDevice(0));
-    hipDeviceProp_t prop;
-    HIPC(hipGetDeviceProperties(&prop, 0));
-    S->cus = prop.multiProcessorCount;
-    const size_t lds_per_cu = prop.maxSharedMemoryPerMultiProcessor ? prop.maxSharedMemoryPerMultiProcessor
-                                                                    : 160 * 1024;
-    S->tu = resolve(&S->cfg.tuning, S->cus, lds_per_cu);
-    if (S->tu.side < -1 || S->tu.side > 1) return fail(MBE_EINVAL, "bad side");
-    if (S->tu.root_begin || S->tu.root_end || S->tu.root_stride > 1)
-        return fail(MBE_EINVAL, "root subsets are not served");
-    mbe_status s = ensure_lds(S->tu.lds_words);
-    if (s != MBE_OK) return s;
-    for (uint32_t i = 0; i < S->cfg.lanes; i++) {
-        S->lanes.push_back(std::make_unique<Lane>());
-        s = make_lane(S.get(), i, *S->lanes.back());
-        if (s != MBE_OK) {
-            for (auto& L : S->lanes) free_lane(*L);
-            return s;
-        }
+    if (!S->cfg.data_bytes) S->cfg.data_bytes = (uint64_t)4 << 30;
+    if (!S->cfg.staging_bytes) S->cfg.staging_bytes = (uint64_t)1 << 30;
+    mbe_status s = make_server(S.get());
+    if (s != MBE_OK) {
+        free_server(S.get());
+        return s;
     }
     mbe_server* raw = S.release();
     raw->th = std::thread([raw] { raw->run(); });
@@ -351,11 +455,22 @@ mbe_status mbe_server_submit(mbe_server* S, const mbe_graph* g, mbe_variant vari
     r->t_submit = now_ms();
     mbe::prepare(g->g, S->tu.side, r->prep);
     r->variant = (int)variant;
-    const uint64_t stack = root_bound_words(r->prep) * 8 *
-                           std::min<uint64_t>(r->prep.n_roots, S->tu.workers);
-    r->footprint = packed_bytes(r->prep) + stack;
+    const uint64_t bound = root_bound_words(r->prep);
+    r->footprint = packed_bytes(r->prep) +
+                   bound * 8 * std::min<uint64_t>(r->prep.n_roots, S->tu.workers);
+    r->words = packed_bytes(r->prep) / 8;
+    // the kernel serves one variant (cuMBE); requests for another variant,
+    // or too big for its arenas and tables, take the engine path
+    r->oversize = bound > S->tu.arena_words || r->prep.nR > S->cfg.max_nr ||
+                  r->words * 8 > S->cfg.data_bytes / 4 || r->words * 8 > S->cfg.staging_bytes / 4 ||
+                  variant != MBE_CUMBE;
+    if (r->oversize) {
+        r->graph = std::make_unique<mbe_graph>();
+        r->graph->g = g->g;
+    }
     std::lock_guard<std::mutex> lk(S->mu);
     if (S->stop) return fail(MBE_EINVAL, "server is shutting down");
+    if (S->failed) return fail(MBE_EHIP, "serving kernel failed: " + S->fail_msg);
     S->st.submitted++;
     if (r->footprint > S->cfg.hbm_budget) {
         S->st.rejected++;
@@ -365,12 +480,12 @@ mbe_status mbe_server_submit(mbe_server* S, const mbe_graph* g, mbe_variant vari
         S->st.rejected++;
         return fail(MBE_EBUSY, "server queue full");
     }
-    r->id = S->next_id++;
+    r->id = S->st.submitted;
     S->queue.push_back(r);
     S->live[r->id] = r;
     S->st.queued++;
     *ticket = r->id;
-    S->work_cv.notify_one();
+    S->work_cv.notify_all();
     return MBE_OK;
 }
 
@@ -408,7 +523,7 @@ void mbe_server_destroy(mbe_server* S) {
         S->work_cv.notify_all();
     }
     if (S->th.joinable()) S->th.join();
-    for (auto& L : S->lanes) free_lane(*L);
+    free_server(S);
     delete S;
 }
 

@@ -140,7 +140,7 @@ def main() -> None:
     ap.add_argument("--rates", default="200,1000,2000")
     ap.add_argument("--duration", type=float, default=60.0)
     ap.add_argument("--large-frac", type=float, default=0.01)
-    ap.add_argument("--large", default="200000,100000,1500000,2.0,0",
+    ap.add_argument("--large", default="200000,100000,1500000,2.0,128",
                     help="n_u,n_v,edges,gamma,max_deg of the large class")
     ap.add_argument("--large-seeds", default="11,12")
     args = ap.parse_args()

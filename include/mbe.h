@@ -169,38 +169,43 @@ void mbe_batch_plan_free(mbe_batch_plan* plan);
 /* A request loop with continuous batching. No reference counterpart (the
  * reference solves one graph per process, src/main.cu:26-234).
  *
- * Submitted graphs are prepared on the caller's thread and queued. A
- * scheduler thread admits queued requests in arrival order while the sum of
- * admitted footprints stays within hbm_budget -- footprint = packed CSR bytes
- * plus worst-case stack (the largest local problem any root can build, in
- * bytes, times the workers that can hold one of its roots at once) -- and
- * starts them on the first free lane: a stream with its own resident buffers
- * and one persistent launch over all roots of the requests it took. Lanes
- * run concurrently on the device, so requests that arrive while others run
- * start as soon as a lane and budget are free, and their work overlaps on
- * the GPU. Each request completes as soon as its own last task does (the
- * kernel raises a per-request flag in host-mapped memory), not at the end of
- * its launch. Results are bit-identical to mbe_solve. The server object is
+ * One persistent kernel (one generation) runs while requests arrive. Submit
+ * prepares the graph on the caller's thread and queues it. A scheduler
+ * thread admits queued requests in arrival order while the sum of admitted
+ * footprints stays within hbm_budget -- footprint = packed CSR bytes plus
+ * worst-case stack (the largest local problem any of its roots can build,
+ * times the workers that can hold one of its roots at once) -- packs the CSR
+ * into pinned staging memory, reserves its place in the device data ring and
+ * publishes a mailbox record. An idle wave of the running kernel copies the
+ * CSR in and appends the request's roots to the shared task space, so a
+ * request starts while earlier ones are still running and its subtrees share
+ * the work-stealing deque with theirs. A request completes when its own last
+ * task does: the kernel raises its host-mapped completion flag. Requests for
+ * another variant than cuMBE, or whose local problems exceed the serving
+ * workers' arenas, run through mbe_solve in their turn, between two kernel
+ * generations. Results are bit-identical to mbe_solve. The server object is
  * internally synchronised; submit and poll may be called from any thread. */
 typedef struct mbe_server mbe_server;
 
 typedef struct {
-    uint32_t lanes;          /* concurrent launch lanes; 0 = 4 */
-    uint32_t max_batch;      /* requests per launch; 0 = 256 */
+    uint32_t table_cap;      /* requests per kernel generation; 0 = 1 << 18 */
     uint32_t queue_cap;      /* queued (unadmitted) requests; 0 = 1 << 20 */
+    uint32_t max_nr;         /* largest enumerated side the kernel serves; 0 = 1 << 17 */
     uint32_t reserved;
     uint64_t hbm_budget;     /* admission budget in bytes; 0 = 64 GiB */
-    uint64_t lane_data_bytes;/* resident CSR capacity per lane; 0 = 256 MiB */
-    mbe_tuning tuning;       /* launch tuning of every lane (NULL fields = defaults) */
+    uint64_t data_bytes;     /* device CSR ring; 0 = 4 GiB */
+    uint64_t staging_bytes;  /* pinned staging ring; 0 = 1 GiB */
+    mbe_tuning tuning;       /* launch tuning of the serving kernel (0 fields =
+                                defaults; arena_words defaults to 1 << 20) */
 } mbe_server_config;
 
 typedef struct {
     mbe_status status;       /* MBE_OK, or the request's failure */
-    uint32_t lane;           /* lane that ran it */
+    uint32_t route;          /* 0 = serving kernel, 1 = mbe_solve path */
     uint64_t count;
     uint64_t hash;
     uint64_t footprint;      /* admitted bytes */
-    double queued_ms;        /* submit -> launch start */
+    double queued_ms;        /* submit -> published to the kernel */
     double latency_ms;       /* submit -> completion seen by the scheduler */
 } mbe_request_result;
 
@@ -221,8 +226,8 @@ mbe_status mbe_server_submit(mbe_server* s, const mbe_graph* g, mbe_variant vari
  * if it has not finished; MBE_EINVAL for an unknown ticket. */
 mbe_status mbe_server_poll(mbe_server* s, uint64_t ticket, int wait_ms, mbe_request_result* out);
 mbe_status mbe_server_get_stats(mbe_server* s, mbe_server_stats* out);
-/* Finish every admitted and queued request, stop the scheduler, release
- * the device buffers. */
+/* Finish every admitted and queued request, close the kernel's mailbox,
+ * wait for the kernel to drain, release the device buffers. */
 void mbe_server_destroy(mbe_server* s);
 
 #ifdef __cplusplus

@@ -1,6 +1,8 @@
 """GPU tests of the serving loop (mbe_server_*): every request's count and
-hash is the same alone and in a mix of concurrent requests, equals the
-oracle's golden vector, and survives arena-overflow passes; admission
+hash is the same alone and in a mix of concurrent requests and equals the
+oracle's golden vector; requests the serving kernel does not take (other
+variants, local problems beyond its arenas) come back exact from the engine
+path; kernel generations rotate when the request table fills; admission
 rejects what the HBM budget cannot hold.
 
 Goldens: tests/golden/batch_c3.csv (the oracle per seed of the batched
@@ -47,7 +49,7 @@ def collect(srv, tickets, timeout_s=120):
 def test_alone_equals_mix_equals_golden():
     rows = golden_rows(50_000_000, 48) + golden_rows(2_000_000_000, 4, min_work=200_000_000)
     graphs = {s: M.Graph.batch_graph(s) for s, _, _ in rows}
-    with M.Server(lanes=4, max_batch=16) as srv:
+    with M.Server() as srv:
         alone = {}
         for s, _, _ in rows:
             say(f"alone {s}")
@@ -81,23 +83,32 @@ def test_alone_equals_mix_equals_golden():
     assert st["bytes_in_use"] == 0
 
 
-def test_overflow_passes_and_variants():
+def test_engine_path_variants_and_rotation():
     rows = golden_rows(20_000_000, 12, min_work=1_000_000)
     graphs = [(s, c, h, M.Graph.batch_graph(s)) for s, c, h in rows]
-    with M.Server(lanes=2, max_batch=8, tuning={"arena_words": 1024, "lds_words": 512}) as srv:
+    # a table of 5 requests per generation forces several generations; the
+    # small arena sends the larger requests to the engine path
+    with M.Server(table_cap=5, tuning={"arena_words": 1 << 14}) as srv:
         tickets = [(s, c, h, v, srv.submit(g, v)) for (s, c, h, g) in graphs
                    for v in ("cuMBE", "noES")]
-        say("overflow + variants")
+        say("engine path + variants + rotation")
+        routes = set()
         for s, c, h, v, t in tickets:
             r = srv.poll(t, wait_ms=120_000)
             assert r is not None and r.status == 0, (s, v)
             assert (r.count, r.hash) == (c, h), (s, v)
+            if v == "noES":
+                assert r.route == 1
+            routes.add(r.route)
+        st = srv.stats()
+    assert routes == {0, 1}
+    assert st["launches"] >= 2 and st["completed"] == len(tickets)
 
 
 def test_admission_budget():
     small = M.Graph.batch_graph(golden_rows(100_000, 1)[0][0])
     big = M.Graph.chung_lu(20000, 8000, 200000, 2.0, 2.0, 0, 5)
-    with M.Server(lanes=1, hbm_budget=64 << 20) as srv:
+    with M.Server(hbm_budget=64 << 20) as srv:
         with pytest.raises(M.MbeError) as e:
             srv.submit(big)
         assert e.value.status == -5
