@@ -58,6 +58,11 @@ TUNING_KEY = "c1_roots400"
 TUNING = dict(lds_words=4096, waves_per_cu=4, donate_interval=64)
 
 
+def note(msg):
+    """Progress on stderr (the JSON line alone goes to stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def golden(name):
     return json.load(open(os.path.join(ROOT, "tests", "golden", name)))
 
@@ -77,6 +82,7 @@ def batch_leg(variant, reps=3):
     """Batched plan vs one launch per graph on the golden-known graphs."""
     rows = c3_rows()
     gs = [M.Graph.batch_graph(s) for s, _, _ in rows]
+    note(f"batch leg: plan over {len(gs)} graphs")
     plan = M.BatchPlan(gs, variant)
     plan.run()  # warm-up (first replay)
     best = None
@@ -90,12 +96,15 @@ def batch_leg(variant, reps=3):
             best = (wall, tot)
     plan.close()
     wall_b, tot = best
+    note(f"batch leg: batched {wall_b:.2f} s; one launch per graph next")
     t0 = time.perf_counter()
     kms = 0.0
-    for (s, c, h), g in zip(rows, gs):
+    for i, ((s, c, h), g) in enumerate(zip(rows, gs)):
         r = M.solve(g, variant)
         kms += r.kernel_ms
         ok &= (r.count, r.hash) == (c, h)
+        if i % 500 == 499:
+            note(f"batch leg: {i + 1} graphs one per launch, {time.perf_counter() - t0:.1f} s")
     wall_p = time.perf_counter() - t0
     n_bic = sum(c for _, c, _ in rows)
     return {
@@ -119,7 +128,8 @@ def serving_leg(duration, rates):
     pick = sorted(random.Random(1).sample(seeds, 512))
     small = {s: M.Graph.batch_graph(s) for s in pick}
     large = {s: M.Graph.chung_lu(*LARGE, s) for s in LARGE_SEEDS}
-    return SB.run(small, large, rates, duration, 0.01)
+    note(f"serving leg: rates {rates}, {duration} s each")
+    return SB.run(small, large, rates, duration, 0.01, progress=note)
 
 
 def traffic_from_profiles():
@@ -187,6 +197,7 @@ def main():
 
     for _ in range(args.warmup):
         M.solve(g, args.variant, tun)
+    note("timed steps")
     count = hsh = nodes = words = build = 0
     kernel_ms = wall_ms = 0.0
     launches = 0
@@ -251,6 +262,7 @@ def main():
         },
         "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(g, base),
     }
+    note(f"timed steps done: {kernel_ms:.0f} ms")
     if not args.no_batch:
         line["batch"] = batch_leg(args.variant)
     if not args.no_serving:

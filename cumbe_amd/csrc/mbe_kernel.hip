@@ -60,8 +60,6 @@ constexpr uint32_t kLdsHdr = (kLdsSec + kSecCount + 7) & ~7u;
 static_assert(kLdsHdr <= kLdsHeaderWords, "LDS header larger than the host reserves");
 constexpr uint32_t kArenaHdr = 0;                // arena words before problems
 constexpr uint32_t kLdsFramesMin = 24;           // frames an LDS tier must hold
-constexpr uint32_t kColVecMax = 8;               // Wc up to which columns are
-                                                 // scanned lane-per-L'-bit
 
 enum Variant : int { kCuMBE = 0, kNoRS = 1, kNoES = 2, kNoWS = 3 };
 
@@ -80,6 +78,11 @@ __device__ __forceinline__ u64 rdlane64(u64 v, uint32_t l) {
     return ((u64)hi << 32) | lo;
 }
 __device__ __forceinline__ u64 lanemask_lt() { return (1ull << lane()) - 1; }
+__device__ __forceinline__ u64 shfl_xor64(u64 v, uint32_t m) {
+    const uint32_t lo = __shfl_xor((int)(uint32_t)v, (int)m);
+    const uint32_t hi = __shfl_xor((int)(uint32_t)(v >> 32), (int)m);
+    return ((u64)hi << 32) | lo;
+}
 
 // Wave reductions with DPP: row_shr 1,2,4,8 builds an inclusive scan inside
 // each 16-lane row, row_bcast:15 and row_bcast:31 carry it across rows, and
@@ -116,24 +119,6 @@ __device__ __forceinline__ u64 red_add64(u64 v) {
     v += MBE_DPP64(0ull, v, 0x118, 0xf);
     v += MBE_DPP64(0ull, v, 0x142, 0xa);
     v += MBE_DPP64(0ull, v, 0x143, 0xc);
-    return rdlane64(v, 63);
-}
-__device__ __forceinline__ u64 red_and64(u64 v) {
-    v &= MBE_DPP64(~0ull, v, 0x111, 0xf);
-    v &= MBE_DPP64(~0ull, v, 0x112, 0xf);
-    v &= MBE_DPP64(~0ull, v, 0x114, 0xf);
-    v &= MBE_DPP64(~0ull, v, 0x118, 0xf);
-    v &= MBE_DPP64(~0ull, v, 0x142, 0xa);
-    v &= MBE_DPP64(~0ull, v, 0x143, 0xc);
-    return rdlane64(v, 63);
-}
-__device__ __forceinline__ u64 red_or64(u64 v) {
-    v |= MBE_DPP64(0ull, v, 0x111, 0xf);
-    v |= MBE_DPP64(0ull, v, 0x112, 0xf);
-    v |= MBE_DPP64(0ull, v, 0x114, 0xf);
-    v |= MBE_DPP64(0ull, v, 0x118, 0xf);
-    v |= MBE_DPP64(0ull, v, 0x142, 0xa);
-    v |= MBE_DPP64(0ull, v, 0x143, 0xc);
     return rdlane64(v, 63);
 }
 
@@ -329,49 +314,63 @@ __device__ uint32_t select_x(const Local& S, u64* F, uint32_t* premin_lvl) {
     uint32_t best = kInf, arg = 0;
     bool nostop = (V == kNoES);
     bool done = false;
-    // four P words per step so that their row gathers are in flight together;
-    // the ES scan then walks them in descending order
-    for (int w0 = (int)S.Wc - 1; w0 >= 0 && !done; w0 -= 4) {
-        u64 pw[4];
-        uint32_t c[4];
+    // P words 64 at a time, highest first (lane i holds word top - i); the
+    // nonzero ones are taken four per step so that their row gathers are in
+    // flight together, and the ES scan walks them in descending order
+    for (int top = (int)S.Wc - 1; top >= 0 && !done; top -= 64) {
+        const int myw = top - (int)lane();
+        const u64 mine = myw >= 0 ? P[myw] : 0;
+        u64 nz = __ballot(mine != 0);
+        while (nz && !done) {
+            u64 pw[4];
+            int wi[4];
+            uint32_t c[4];
 #pragma unroll
-        for (int s = 0; s < 4; s++) {
-            const int w = w0 - s;
-            pw[s] = w >= 0 ? uni64(P[w]) : 0;
-            c[s] = kInf;
-        }
-#pragma unroll
-        for (int s = 0; s < 4; s++) {
-            if ((pw[s] >> lane()) & 1) {
-                const u64* r = S.rows + (size_t)((w0 - s) * 64 + lane()) * S.W;
-                uint32_t cc = 0;
-                for (uint32_t i = 0; i < S.W; i++) cc += __popcll(r[i] & L[i]);
-                c[s] = cc;
+            for (int s = 0; s < 4; s++) {
+                if (nz) {
+                    const uint32_t li = __builtin_ctzll(nz);
+                    nz &= nz - 1;
+                    pw[s] = rdlane64(mine, li);
+                    wi[s] = top - (int)li;
+                } else {
+                    pw[s] = 0;
+                    wi[s] = 0;
+                }
+                c[s] = kInf;
             }
-        }
 #pragma unroll
-        for (int s = 0; s < 4; s++) {
-            if (done || !pw[s]) continue;
-            const int w = w0 - s;
-            const bool valid = (pw[s] >> lane()) & 1;
-            if (!nostop) {
-                const u64 m = __ballot(valid && c[s] <= pm);
-                if (m) {
-                    const uint32_t sl = 63 - __builtin_clzll(m);
-                    const uint32_t cs = rdlane32(c[s], sl);
-                    if (cs == pm) {
-                        best = cs;
-                        arg = w * 64 + sl;
-                        done = true;
-                        continue;
-                    }
-                    nostop = true;
+            for (int s = 0; s < 4; s++) {
+                if ((pw[s] >> lane()) & 1) {
+                    const u64* r = S.rows + (size_t)(wi[s] * 64 + lane()) * S.W;
+                    uint32_t cc = 0;
+                    for (uint32_t i = 0; i < S.W; i++) cc += __popcll(r[i] & L[i]);
+                    c[s] = cc;
                 }
             }
-            const uint32_t cm = red_min32(c[s]);
-            if (cm < best) {
-                best = cm;
-                arg = w * 64 + (63 - __builtin_clzll(__ballot(c[s] == cm)));
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                if (done || !pw[s]) continue;
+                const int w = wi[s];
+                const bool valid = (pw[s] >> lane()) & 1;
+                if (!nostop) {
+                    const u64 m = __ballot(valid && c[s] <= pm);
+                    if (m) {
+                        const uint32_t sl = 63 - __builtin_clzll(m);
+                        const uint32_t cs = rdlane32(c[s], sl);
+                        if (cs == pm) {
+                            best = cs;
+                            arg = w * 64 + sl;
+                            done = true;
+                            continue;
+                        }
+                        nostop = true;
+                    }
+                }
+                const uint32_t cm = red_min32(c[s]);
+                if (cm < best) {
+                    best = cm;
+                    arg = w * 64 + (63 - __builtin_clzll(__ballot(c[s] == cm)));
+                }
             }
         }
     }
@@ -418,8 +417,9 @@ __device__ uint32_t make_lprime(const Local& S, const u64* L, uint32_t x, u64* L
 }
 
 // Reverse-scan maximality check and expansion (src/mbe_noWS.cuh:215-289):
-// AND/OR of the L' columns. Small Wc: one lane per L' bit, one word at a
-// time; large Wc: one lane per column word, walking the L' bits.
+// AND/OR of the L' columns. Wc <= 64: lanes split into groups of L' bits
+// times column words, then a butterfly over the groups; larger Wc: one lane
+// per column word, walking the L' bits.
 __device__ NodeOut process_rs(const Local& S, u64* F, u64* Fc, uint32_t nl) {
     const u64* P = F + kFrameHdr + S.W;
     const u64* Q = P + S.Wc;
@@ -429,36 +429,63 @@ __device__ NodeOut process_rs(const Local& S, u64* F, u64* Fc, uint32_t nl) {
     u64* Xc = Qc + S.Wc;
     NodeOut o{true, false, 0, 0, 0, 0, 0};
     u64 abh = 0;
-    if (S.Wc <= kColVecMax) {
-        for (uint32_t w = 0; w < S.Wc; w++) {
-            u64 a = ~0ull, ob = 0;
-            for (uint32_t c0 = 0; c0 < nl; c0 += 64) {
-                const uint32_t l = c0 + lane();
-                u64 ca = ~0ull, co = 0;
-                if (l < nl) {
-                    const u64 c = S.cols[(size_t)S.lbits[l] * S.Wc + w];
-                    ca = c;
-                    co = c;
-                }
-                a &= red_and64(ca);
-                ob |= red_or64(co);
+    if (S.Wc <= 64) {
+        // lane = group * Wp + word (Wp = Wc rounded up to a power of two):
+        // each of the 64 / Wp groups AND/ORs every (64 / Wp)-th L' column
+        // with independent loads, then a butterfly over the groups leaves
+        // the totals in every lane of the word
+        const uint32_t Wc = S.Wc;
+        const uint32_t lg = Wc > 1 ? 32 - __builtin_clz(Wc - 1) : 0;
+        const uint32_t Wp = 1u << lg, G = 64u >> lg;
+        const uint32_t w = lane() & (Wp - 1), gi = lane() >> lg;
+        const bool valid = w < Wc;
+        u64 a = ~0ull, ob = 0;
+        if (valid) {
+            uint32_t l = gi;
+            for (; l + 3 * G < nl; l += 4 * G) {
+                const uint32_t j0 = S.lbits[l], j1 = S.lbits[l + G];
+                const uint32_t j2 = S.lbits[l + 2 * G], j3 = S.lbits[l + 3 * G];
+                const u64 c0 = S.cols[(size_t)j0 * Wc + w], c1 = S.cols[(size_t)j1 * Wc + w];
+                const u64 c2 = S.cols[(size_t)j2 * Wc + w], c3 = S.cols[(size_t)j3 * Wc + w];
+                a &= c0 & c1 & c2 & c3;
+                ob |= c0 | c1 | c2 | c3;
             }
-            const u64 q = uni64(Q[w]), p = uni64(P[w] | X[w]);
-            if (a & q) {
-                o.maximal = false;
-                break;
+            for (; l < nl; l += G) {
+                const u64 c = S.cols[(size_t)S.lbits[l] * Wc + w];
+                a &= c;
+                ob |= c;
             }
+        }
+        for (uint32_t m = Wp; m < 64; m <<= 1) {
+            a &= shfl_xor64(a, m);
+            ob |= shfl_xor64(ob, m);
+        }
+        const bool mine = valid && gi == 0;
+        const u64 q = mine ? Q[w] : 0, p = mine ? (P[w] | X[w]) : 0;
+        const bool nonmax = mine && (a & q) != 0;
+        o.maximal = __ballot(nonmax) == 0;
+        if (o.maximal) {
             const u64 absd = a & p, pn = ob & p & ~a, qn = ob & q;
-            if (lane() == 0) {
+            if (mine) {
                 Pc[w] = pn;
                 Qc[w] = qn;
                 Xc[w] = 0;
             }
-            o.ab += __popcll(absd);
-            o.pn += __popcll(pn);
-            o.qn += __popcll(qn);
-            o.un += __popcll(pn | qn);
-            if ((absd >> lane()) & 1) abh += S.hc[w * 64 + lane()];
+            // four counts of at most 4096 each, packed into one reduction
+            const u64 packed = (u64)__popcll(absd) | ((u64)__popcll(pn) << 16) |
+                               ((u64)__popcll(qn) << 32) | ((u64)__popcll(pn | qn) << 48);
+            const u64 tot = red_add64(packed);
+            o.ab = (uint32_t)(tot & 0xFFFF);
+            o.pn = (uint32_t)((tot >> 16) & 0xFFFF);
+            o.qn = (uint32_t)((tot >> 32) & 0xFFFF);
+            o.un = (uint32_t)(tot >> 48);
+            u64 am = __ballot(mine && absd != 0);
+            while (am) {  // absorbed candidates' hash terms, one word at a time
+                const uint32_t ww = __builtin_ctzll(am);
+                am &= am - 1;
+                const u64 aw = rdlane64(absd, ww);
+                if ((aw >> lane()) & 1) abh += S.hc[ww * 64 + lane()];
+            }
         }
     } else {
         bool nonmax = false;

@@ -84,13 +84,15 @@ def poisson_run(srv: Server, small: Dict[str, Graph], large: Dict[str, Graph], r
 
 def run(small: Dict[str, Graph], large: Dict[str, Graph], rates: Sequence[float],
         duration: float, large_frac: float, seed: int = 7,
-        server_kw: Optional[dict] = None) -> dict:
+        server_kw: Optional[dict] = None, progress=lambda msg: None) -> dict:
     out = {"duration_s": duration, "large_frac": large_frac, "rates": {}}
     with Server(**(server_kw or {})) as srv:
         alone = isolated_results(srv, {**{("s", k): g for k, g in small.items()},
                                        **{("l", k): g for k, g in large.items()}})
+        progress(f"serving: {len(alone)} graphs served alone")
         for rate in rates:
             r = poisson_run(srv, small, large, rate, duration, large_frac, seed)
+            progress(f"serving: rate {rate} done in {r['wall_s']:.1f} s")
             lat = {"small": [], "large": []}
             iso = {"small": [], "large": []}
             mismatches = 0
