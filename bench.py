@@ -179,12 +179,18 @@ def main():
     ap.add_argument("--serve-duration", type=float, default=5.0,
                     help="seconds of arrivals per rate (60 for the full workload)")
     ap.add_argument("--serve-rates", default="200,1000,2000")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="run only the batch leg and print its object (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the engine "
               f"shards a graph by root interleaving, see mbe_tuning.root_stride, but this "
               f"benchmark runs on a single MI355X)", file=sys.stderr)
         sys.exit(2)
+    if args.batch_only:
+        M.solve(M.Graph.chung_lu(200, 100, 600, 2.1, 2.1, 0, 1), args.variant)
+        print(json.dumps({"batch": batch_leg(args.variant)}))
+        return
 
     g = M.Graph.chung_lu(WORKLOAD["n_u"], WORKLOAD["n_v"], WORKLOAD["m"], WORKLOAD["gamma_u"],
                          WORKLOAD["gamma_v"], WORKLOAD["max_deg"], WORKLOAD["seed"])
