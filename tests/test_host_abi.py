@@ -177,3 +177,29 @@ def test_root_shards_add_up():
     parts = [O.solve_roots(n_u, n_v, off, adj, begin=k, end=0, stride=4) for k in range(4)]
     assert sum(p.count for p in parts) == whole.count
     assert sum(p.hash for p in parts) & ((1 << 64) - 1) == whole.hash
+
+
+def test_edge_list_conversion_matches_reference_gen_bi(tmp_path):
+    """`mbe convert` against the reference's own converter (data/gen_bi.cpp,
+    compiled from its source into oracle/_ref by `make -C oracle ref`): the
+    same edge file gives the same .bi graph (gen_bi writes each row in hash-set
+    order, so lists are compared sorted)."""
+    ref = os.path.join(ROOT, "oracle", "_ref", "gen_bi")
+    if not os.path.exists(ref):
+        pytest.skip("oracle/_ref/gen_bi not built (the reference is not present)")
+    rng = np.random.default_rng(5)
+    lines = ["% bip", "% 1 2 3"]
+    for _ in range(3000):  # duplicates, sparse ids, an extra word per edge
+        lines.append(f"{int(rng.integers(0, 400)) * 7 + 3} {int(rng.integers(0, 150)) * 11} "
+                     f"{int(rng.integers(1, 9))}")
+    src = tmp_path / "edges.txt"
+    src.write_text("\n".join(lines) + "\n")
+    theirs = tmp_path / "ref.bi"
+    subprocess.run([ref, str(src), str(theirs), "6", "1"], check=True, capture_output=True,
+                   timeout=120)
+    ours = tmp_path / "ours.bi"
+    M.Graph.from_edge_list(str(src), skip_words=6, skip_per_edge=1).write_bi(str(ours))
+    assert theirs.read_text().split("\n")[0] == ours.read_text().split("\n")[0]
+    a, b = M.Graph.load_bi(str(theirs)).csr(), M.Graph.load_bi(str(ours)).csr()
+    assert a[:2] == b[:2]
+    assert np.array_equal(a[2], b[2]) and np.array_equal(a[3], b[3])
