@@ -115,3 +115,28 @@ def test_admission_budget():
         r = srv.poll(srv.submit(small), wait_ms=60_000)
         assert r is not None and r.status == 0 and r.footprint <= 64 << 20
         assert srv.stats()["rejected"] == 1
+
+
+def test_large_class_exact_alone_and_served():
+    """The serving workload's large class (200,000 x 100,000, 1.5M edges,
+    degrees capped at 128) against the oracle, through mbe_solve and through
+    the server together with small requests."""
+    import json
+    c = json.load(open(os.path.join(ROOT, "tests", "golden", "configs.json")))["serving_large"]
+    graphs = {e["seed"]: M.Graph.chung_lu(**c["graph"], seed=e["seed"]) for e in c["seeds"]}
+    gold = {e["seed"]: (e["count"], int(e["hash"], 16)) for e in c["seeds"]}
+    for s, g in graphs.items():
+        say(f"large {s} alone")
+        r = M.solve(g)
+        assert (r.count, r.hash) == gold[s], s
+    small = golden_rows(5_000_000, 20)
+    with M.Server() as srv:
+        t_large = {s: srv.submit(g) for s, g in graphs.items()}
+        t_small = {s: srv.submit(M.Graph.batch_graph(s)) for s, _, _ in small}
+        say("large + small served together")
+        for s, t in t_large.items():
+            r = srv.poll(t, wait_ms=120_000)
+            assert r is not None and r.status == 0 and (r.count, r.hash) == gold[s], s
+        for s, c0, h0 in small:
+            r = srv.poll(t_small[s], wait_ms=120_000)
+            assert r is not None and (r.count, r.hash) == (c0, h0), s
