@@ -347,29 +347,42 @@ __device__ uint32_t select_x(const Local& S, u64* F, uint32_t* premin_lvl) {
                     c[s] = cc;
                 }
             }
+            // ES: the first candidate in scan order (words in order, high
+            // lanes first) whose count is at most the level's previous
+            // minimum stops the scan if it equals it; one below it ends early
+            // stopping. Counts of absent candidates are kInf.
+            if (!nostop) {
 #pragma unroll
-            for (int s = 0; s < 4; s++) {
-                if (done || !pw[s]) continue;
-                const int w = wi[s];
-                const bool valid = (pw[s] >> lane()) & 1;
-                if (!nostop) {
-                    const u64 m = __ballot(valid && c[s] <= pm);
-                    if (m) {
-                        const uint32_t sl = 63 - __builtin_clzll(m);
-                        const uint32_t cs = rdlane32(c[s], sl);
-                        if (cs == pm) {
-                            best = cs;
-                            arg = w * 64 + sl;
-                            done = true;
-                            continue;
+                for (int s = 0; s < 4; s++) {
+                    const u64 m = __ballot(c[s] <= pm);
+                    if (!m) continue;
+                    const uint32_t sl = 63 - __builtin_clzll(m);
+                    if (rdlane32(c[s], sl) == pm) {  // earlier words hold no count <= pm
+                        done = true;
+                        if (best > pm) {
+                            best = pm;
+                            arg = wi[s] * 64 + sl;
                         }
+                    } else {
                         nostop = true;
                     }
+                    break;
                 }
-                const uint32_t cm = red_min32(c[s]);
+            }
+            if (!done) {
+                // one reduction for the group; ties go to the first word in
+                // scan order and its highest lane
+                uint32_t mn = c[0];
+#pragma unroll
+                for (int s = 1; s < 4; s++) mn = min(mn, c[s]);
+                const uint32_t cm = red_min32(mn);
                 if (cm < best) {
                     best = cm;
-                    arg = w * 64 + (63 - __builtin_clzll(__ballot(c[s] == cm)));
+#pragma unroll
+                    for (int s = 3; s >= 0; s--) {
+                        const u64 m = __ballot(c[s] == cm);
+                        if (m) arg = wi[s] * 64 + (63 - __builtin_clzll(m));
+                    }
                 }
             }
         }
@@ -456,15 +469,14 @@ __device__ NodeOut process_rs(const Local& S, u64* F, u64* Fc, uint32_t nl) {
                 ob |= c;
             }
         }
-        for (uint32_t m = Wp; m < 64; m <<= 1) {
-            a &= shfl_xor64(a, m);
-            ob |= shfl_xor64(ob, m);
-        }
+        for (uint32_t m = Wp; m < 64; m <<= 1) a &= shfl_xor64(a, m);
         const bool mine = valid && gi == 0;
-        const u64 q = mine ? Q[w] : 0, p = mine ? (P[w] | X[w]) : 0;
+        const u64 q = mine ? Q[w] : 0;
         const bool nonmax = mine && (a & q) != 0;
         o.maximal = __ballot(nonmax) == 0;
-        if (o.maximal) {
+        if (o.maximal) {  // O is combined only where it is used
+            for (uint32_t m = Wp; m < 64; m <<= 1) ob |= shfl_xor64(ob, m);
+            const u64 p = mine ? (P[w] | X[w]) : 0;
             const u64 absd = a & p, pn = ob & p & ~a, qn = ob & q;
             if (mine) {
                 Pc[w] = pn;
@@ -488,9 +500,27 @@ __device__ NodeOut process_rs(const Local& S, u64* F, u64* Fc, uint32_t nl) {
             }
         }
     } else {
+        // First pass: is some Q candidate adjacent to all of L'? L' is
+        // usually tiny at non-maximal nodes and covering candidates are
+        // plentiful, so this stops at the first 64-word chunk holding one.
         bool nonmax = false;
+        for (uint32_t w0 = 0; w0 < S.Wc && !nonmax; w0 += 64) {
+            const uint32_t w = w0 + lane();
+            u64 a = w < S.Wc ? Q[w] : 0;
+            uint32_t l = 0;
+            for (; l + 4 <= nl && a; l += 4) {
+                const uint32_t j0 = S.lbits[l], j1 = S.lbits[l + 1];
+                const uint32_t j2 = S.lbits[l + 2], j3 = S.lbits[l + 3];
+                a &= S.cols[(size_t)j0 * S.Wc + w] & S.cols[(size_t)j1 * S.Wc + w] &
+                     S.cols[(size_t)j2 * S.Wc + w] & S.cols[(size_t)j3 * S.Wc + w];
+            }
+            for (; l < nl && a; l++) a &= S.cols[(size_t)S.lbits[l] * S.Wc + w];
+            nonmax = __ballot(a != 0) != 0;
+        }
+        o.maximal = !nonmax;
+        // second pass, maximal nodes only: A and O over every word
         uint32_t ab = 0, pnc = 0, qnc = 0, unc = 0;
-        for (uint32_t w0 = 0; w0 < S.Wc; w0 += 64) {
+        for (uint32_t w0 = 0; w0 < S.Wc && o.maximal; w0 += 64) {
             const uint32_t w = w0 + lane();
             const bool valid = w < S.Wc;
             u64 a = ~0ull, ob = 0;
@@ -514,7 +544,6 @@ __device__ NodeOut process_rs(const Local& S, u64* F, u64* Fc, uint32_t nl) {
                 }
             }
             const u64 q = valid ? Q[w] : 0, p = valid ? (P[w] | X[w]) : 0;
-            nonmax |= (a & q) != 0;
             const u64 absd = a & p, pn = ob & p & ~a, qn = ob & q;
             if (valid) {
                 Pc[w] = pn;
@@ -532,11 +561,12 @@ __device__ NodeOut process_rs(const Local& S, u64* F, u64* Fc, uint32_t nl) {
                 abh += S.hc[w * 64 + b];
             }
         }
-        o.maximal = __ballot(nonmax) == 0;
-        o.ab = red_add32(ab);
-        o.pn = red_add32(pnc);
-        o.qn = red_add32(qnc);
-        o.un = red_add32(unc);
+        if (o.maximal) {
+            o.ab = red_add32(ab);
+            o.pn = red_add32(pnc);
+            o.qn = red_add32(qnc);
+            o.un = red_add32(unc);
+        }
     }
     o.abh = red_add64(abh);
     o.descend = o.maximal && o.pn != 0;
