@@ -178,7 +178,7 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serve-duration", type=float, default=5.0,
                     help="seconds of arrivals per rate (60 for the full workload)")
-    ap.add_argument("--serve-rates", default="200,1000,2000")
+    ap.add_argument("--serve-rates", default="50,200,1000,2000")
     ap.add_argument("--batch-only", action="store_true",
                     help="run only the batch leg and print its object (for profiling)")
     args = ap.parse_args()

@@ -455,9 +455,11 @@ mbe_status mbe_server_submit(mbe_server* S, const mbe_graph* g, mbe_variant vari
     r->t_submit = now_ms();
     mbe::prepare(g->g, S->tu.side, r->prep);
     r->variant = (int)variant;
+    // footprint: the packed CSR plus the worst-case stack, i.e. the largest
+    // local problem (bit matrices and frame stack) one of its roots can build;
+    // the per-worker arenas that hold those stacks are allocated once
     const uint64_t bound = root_bound_words(r->prep);
-    r->footprint = packed_bytes(r->prep) +
-                   bound * 8 * std::min<uint64_t>(r->prep.n_roots, S->tu.workers);
+    r->footprint = packed_bytes(r->prep) + bound * 8;
     r->words = packed_bytes(r->prep) / 8;
     // the kernel serves one variant (cuMBE); requests for another variant,
     // or too big for its arenas and tables, take the engine path

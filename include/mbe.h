@@ -173,8 +173,8 @@ void mbe_batch_plan_free(mbe_batch_plan* plan);
  * prepares the graph on the caller's thread and queues it. A scheduler
  * thread admits queued requests in arrival order while the sum of admitted
  * footprints stays within hbm_budget -- footprint = packed CSR bytes plus
- * worst-case stack (the largest local problem any of its roots can build,
- * times the workers that can hold one of its roots at once) -- packs the CSR
+ * worst-case stack (the largest local problem, bit matrices and frame stack,
+ * that any of its roots can build) -- packs the CSR
  * into pinned staging memory, reserves its place in the device data ring and
  * publishes a mailbox record. An idle wave of the running kernel copies the
  * CSR in and appends the request's roots to the shared task space, so a

@@ -105,15 +105,45 @@ def test_engine_path_variants_and_rotation():
     assert st["launches"] >= 2 and st["completed"] == len(tickets)
 
 
+def footprint(g):
+    """Admission footprint as the server defines it: packed CSR bytes plus
+    the largest local problem (u64 words) any root can build."""
+    import numpy as np
+    n_u, n_v, off, adj = g.csr()
+    du = np.diff(off).astype(np.int64)
+    dv = np.bincount(adj, minlength=n_v).astype(np.int64)
+    row = np.repeat(np.arange(n_u), du)
+    if n_u <= n_v:
+        nR, nL, dR, dL = n_u, n_v, du, dv
+        s = np.bincount(row, weights=dv[adj] - 1, minlength=n_u)
+    else:
+        nR, nL, dR, dL = n_v, n_u, dv, du
+        s = np.bincount(adj, weights=du[row] - 1, minlength=n_v)
+    d = dR[dR > 0]
+    nc = np.minimum(s[dR > 0].astype(np.int64), nR - 1)
+    W, Wc = (d + 63) // 64, (nc + 63) // 64
+    words = (d + nc + nc * W + d * Wc + (nc + 1) // 2 + 1 + (d + 1) // 2 + 1 +
+             (np.minimum(d, nc) + 2) * (4 + W + 3 * Wc) +
+             (d + 2) // 2 + np.maximum(d, (nc + 1) // 2) + 1)
+    slot = lambda b: (b + 8 + 255) & ~255
+    E = int(off[-1])
+    packed = (slot((nR + 1) * 4) + slot(E * 4) + slot((nL + 1) * 4) + slot(E * 4) +
+              slot(nR * 8) + slot(nL * 8))
+    return packed + 8 * int(words.max())
+
+
 def test_admission_budget():
     small = M.Graph.batch_graph(golden_rows(100_000, 1)[0][0])
     big = M.Graph.chung_lu(20000, 8000, 200000, 2.0, 2.0, 0, 5)
-    with M.Server(hbm_budget=64 << 20) as srv:
+    fs, fb = footprint(small), footprint(big)
+    assert fs < fb
+    budget = (fs + fb) // 2
+    with M.Server(hbm_budget=budget) as srv:
         with pytest.raises(M.MbeError) as e:
             srv.submit(big)
         assert e.value.status == -5
         r = srv.poll(srv.submit(small), wait_ms=60_000)
-        assert r is not None and r.status == 0 and r.footprint <= 64 << 20
+        assert r is not None and r.status == 0 and r.footprint == fs
         assert srv.stats()["rejected"] == 1
 
 
