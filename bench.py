@@ -55,7 +55,7 @@ SAMPLE = dict(begin=1000, end=0, stride=64)
 LARGE = (200000, 100000, 1500000, 2.0, 2.0, 128)
 LARGE_SEEDS = (11, 12)
 TUNING_KEY = "c1_roots400"
-TUNING = dict(lds_words=4096, waves_per_cu=4, donate_interval=64)
+TUNING = dict(lds_words=3072, waves_per_cu=6, donate_interval=64)
 
 
 def note(msg):
@@ -169,8 +169,8 @@ def cpu_baseline(g, gpu_tuning):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--variant", default="cuMBE", choices=list(M.VARIANTS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--root-begin", type=int, default=ROOT_BEGIN)

@@ -1,12 +1,13 @@
 """Autotuner for the enumeration launch parameters.
 
-Picks, per workload key, the per-worker LDS budget (which fixes how many
-single-wave workers a CU holds), the workers per CU, the donation interval of
-the work-stealing deque, the list-versus-bitmap cutoff (|P|+|Q| at or below
-which a node tests candidate rows instead of AND/OR-ing L' columns) and the
-re-localisation ratio. Coordinate descent from a start point; every trial
-runs under the engine's watchdog (a time limit), and a trial that hits it is
-discarded. Results are cached in tuning_cache.json next to this file and
+Picks, per workload key, the per-worker LDS budget together with the workers
+per CU (pairs that fill a CU's LDS), the donation interval of the
+work-stealing deque, the list-versus-bitmap cutoff (|P|+|Q| at or below which
+a node tests candidate rows instead of AND/OR-ing L' columns) and the
+re-localisation ratio. Coordinate descent from a start point: each trial is
+the mean of two runs and a move needs a 5 % gain (single runs of one
+configuration vary by several percent). Every trial runs under the engine's
+watchdog (a time limit), and a trial that hits it is discarded. Results are cached in tuning_cache.json next to this file and
 returned by cached(key).
 
 Usage: python -m cumbe_amd.autotune <workload> [--budget-s S]
@@ -25,15 +26,26 @@ from typing import Callable, Dict, List, Optional
 HERE = os.path.dirname(os.path.abspath(__file__))
 CACHE = os.path.join(HERE, "tuning_cache.json")
 
+# LDS words per worker and workers per CU are searched together: each pair
+# fills a CU's 160 KiB of LDS (more workers hide more latency, more LDS keeps
+# more local problems out of HBM)
+OCCUPANCY = [(6144, 3), (5120, 4), (4096, 4), (3328, 6), (3072, 6), (2560, 8), (2048, 10),
+             (1536, 13)]
 GRID = {
-    "lds_words": [2048, 3072, 4096, 6144, 8192],
-    "waves_per_cu": [2, 3, 4, 5, 6, 8],
+    "occupancy": OCCUPANCY,
     "donate_interval": [16, 64, 256, 1024],
     "list_cutoff": [0, 16, 64, 256],
     "compact_ratio": [2, 4, 0xFFFFFFFF],
 }
-START = {"lds_words": 4096, "waves_per_cu": 4, "donate_interval": 64, "list_cutoff": 0,
-         "compact_ratio": 2}
+START = {"occupancy": (3072, 6), "donate_interval": 64, "list_cutoff": 0, "compact_ratio": 2}
+
+
+def expand(point: dict) -> dict:
+    """A search point as engine tuning (occupancy -> lds_words, waves_per_cu)."""
+    t = {k: v for k, v in point.items() if k != "occupancy"}
+    if "occupancy" in point:
+        t["lds_words"], t["waves_per_cu"] = point["occupancy"]
+    return t
 
 
 def cached(key: str) -> Optional[dict]:
@@ -55,11 +67,20 @@ def _store(key: str, tuning: dict, ms: float, trials: List[dict]) -> None:
 
 
 def tune(key: str, run: Callable[[dict], Optional[float]], budget_s: float,
-         start: Optional[dict] = None) -> dict:
-    """Coordinate descent: run(tuning) returns kernel ms (None if it failed)."""
+         start: Optional[dict] = None, repeats: int = 2, gain: float = 0.05) -> dict:
+    """Coordinate descent over GRID: run(tuning) returns kernel ms (None if it
+    failed), averaged over `repeats` runs; a neighbour replaces the current
+    point when it is faster by more than `gain`. The cached and returned
+    tuning is in engine terms (expand)."""
+    one = run
+
+    def run(t):  # noqa: F811  (mean of repeated runs)
+        ms = [one(t) for _ in range(repeats)]
+        return None if any(m is None for m in ms) else sum(ms) / len(ms)
+
     best = dict(start or START)
-    best_ms = run(best)
-    trials = [{"tuning": dict(best), "ms": best_ms}]
+    best_ms = run(expand(best))
+    trials = [{"tuning": expand(best), "ms": best_ms}]
     t0 = time.time()
     improved = True
     while improved and time.time() - t0 < budget_s:
@@ -69,13 +90,13 @@ def tune(key: str, run: Callable[[dict], Optional[float]], budget_s: float,
                 if v == best[name] or time.time() - t0 > budget_s:
                     continue
                 cand = dict(best, **{name: v})
-                ms = run(cand)
-                trials.append({"tuning": cand, "ms": ms})
+                ms = run(expand(cand))
+                trials.append({"tuning": expand(cand), "ms": ms})
                 print(f"autotune {key}: {name}={v} -> {ms}", flush=True)
-                if ms is not None and (best_ms is None or ms < best_ms * 0.97):
+                if ms is not None and (best_ms is None or ms < best_ms * (1.0 - gain)):
                     best, best_ms, improved = cand, ms, True
-    _store(key, best, best_ms, trials)
-    return best
+    _store(key, expand(best), best_ms, trials)
+    return expand(best)
 
 
 def _workload(key: str):

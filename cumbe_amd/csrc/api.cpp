@@ -78,12 +78,12 @@ Engine& engine() {
 
 Tune resolve(const mbe_tuning* t, int cus, size_t lds_per_cu) {
     Tune r;
-    r.lds_words = (t && t->lds_words) ? t->lds_words : 4096;  // 32 KiB: 5 workers per CU
+    r.lds_words = (t && t->lds_words) ? t->lds_words : 3072;  // 24 KiB: 6 workers per CU
     r.lds_words = std::max<uint32_t>(r.lds_words, mbe::kLdsHeaderWords + 64);
     r.lds_words = std::min<uint32_t>(r.lds_words, (uint32_t)(lds_per_cu / 8));
     // default occupancy: as many single-wave workers per CU as the LDS holds
     uint32_t fit = (uint32_t)std::max<size_t>(1, lds_per_cu / ((size_t)r.lds_words * 8));
-    uint32_t wpc = (t && t->waves_per_cu) ? t->waves_per_cu : std::min<uint32_t>(fit, 4);
+    uint32_t wpc = (t && t->waves_per_cu) ? t->waves_per_cu : std::min<uint32_t>(fit, 8);
     r.workers = (t && t->workers) ? t->workers : (uint32_t)cus * wpc;
     r.arena_words = (t && t->arena_words) ? t->arena_words : (1u << 18);
     r.list_cutoff = (t && t->list_cutoff) ? t->list_cutoff : 0;

@@ -143,17 +143,20 @@ def test_bench_refuses_more_than_one_gpu():
 def test_autotune_coordinate_descent(tmp_path, monkeypatch):
     from cumbe_amd import autotune as A
     monkeypatch.setattr(A, "CACHE", str(tmp_path / "cache.json"))
-    best_point = {"lds_words": 6144, "waves_per_cu": 3, "donate_interval": 16,
-                  "list_cutoff": 64, "compact_ratio": 4}
+    best_point = {"occupancy": (2560, 8), "donate_interval": 16, "list_cutoff": 64,
+                  "compact_ratio": 4}
 
     def cost(t):  # separable bowl with its minimum on the grid, steep enough
         # that every step towards it beats the 3% acceptance threshold
-        return 100.0 * 1.5 ** sum(abs(A.GRID[k].index(t[k]) - A.GRID[k].index(v))
+        p = dict(t)
+        p["occupancy"] = (p.pop("lds_words"), p.pop("waves_per_cu"))
+        return 100.0 * 1.5 ** sum(abs(A.GRID[k].index(p[k]) - A.GRID[k].index(v))
                                   for k, v in best_point.items())
 
     got = A.tune("toy", cost, budget_s=60)
-    assert got == best_point
-    assert A.cached("toy") == best_point
+    assert got == A.expand(best_point)
+    assert got["lds_words"] == 2560 and got["waves_per_cu"] == 8
+    assert A.cached("toy") == got
     data = json.load(open(tmp_path / "cache.json"))
     assert data["toy"]["kernel_ms"] == 100.0
 
