@@ -228,7 +228,8 @@ def main():
     # algorithmic bytes: 8 per adjacency entry of the canonical algorithm
     wl = golden("configs.json")["c1"]["bench_workload"]
     w_alg = wl["work"] if args.root_begin == wl["begin"] else None
-    achieved = (8.0 * w_alg / (kernel_ms / 1000.0) / 1e9) if w_alg else None
+    # W is per launch (one launch per step): divide by one launch's time
+    achieved = (8.0 * w_alg / (launch_ms / 1000.0) / 1e9) if w_alg else None
     bitmap_gbps = 8.0 * (words + build) / (kernel_ms / 1000.0) / 1e9
     traffic = traffic_from_profiles()
     line = {
