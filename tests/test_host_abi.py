@@ -206,3 +206,15 @@ def test_edge_list_conversion_matches_reference_gen_bi(tmp_path):
     a, b = M.Graph.load_bi(str(theirs)).csr(), M.Graph.load_bi(str(ours)).csr()
     assert a[:2] == b[:2]
     assert np.array_equal(a[2], b[2]) and np.array_equal(a[3], b[3])
+
+
+def test_serve_bench_helpers(tmp_path):
+    from cumbe_amd import serve_bench as SB
+    xs = [float(i) for i in range(1, 101)]
+    assert SB.pct(xs, 50) == 50.0 and SB.pct(xs, 99) == 99.0 and SB.pct(xs, 99.9) == 100.0
+    assert SB.pct([], 50) is None
+    assert SB.summary([3.0, 1.0, 2.0]) == {"n": 3, "p50_ms": 2.0, "p99_ms": 3.0, "p999_ms": 3.0}
+    seeds = SB.seeds_from_csv(os.path.join(ROOT, "tests", "golden", "batch_c3.csv"))
+    assert len(seeds) == 3471 and seeds[0] == 1000 and 1006 not in seeds
+    light = SB.seeds_from_csv(os.path.join(ROOT, "tests", "golden", "batch_c3.csv"), max_work=10**6)
+    assert set(light) < set(seeds)
