@@ -395,9 +395,13 @@ mbe_status make_server(mbe_server* S) {
     const size_t lds_per_cu = prop.maxSharedMemoryPerMultiProcessor
                                   ? prop.maxSharedMemoryPerMultiProcessor
                                   : 160 * 1024;
-    // serving arenas default to 8 MiB per worker, so that every root of the
-    // batched size distribution fits the kernel
+    // serving defaults: 8 MiB arenas per worker, so that every root of the
+    // batched size distribution fits the kernel, and four 32 KiB workers per
+    // CU (the large requests' local problems stay in LDS; six 24 KiB workers
+    // raised their isolated latency by about 40 %)
     if (!S->cfg.tuning.arena_words) S->cfg.tuning.arena_words = 1u << 20;
+    if (!S->cfg.tuning.lds_words) S->cfg.tuning.lds_words = 4096;
+    if (!S->cfg.tuning.waves_per_cu && !S->cfg.tuning.workers) S->cfg.tuning.waves_per_cu = 4;
     S->tu = resolve(&S->cfg.tuning, prop.multiProcessorCount, lds_per_cu);
     if (S->tu.side < -1 || S->tu.side > 1) return fail(MBE_EINVAL, "bad side");
     if (S->tu.root_begin || S->tu.root_end || S->tu.root_stride > 1)

@@ -196,7 +196,8 @@ typedef struct {
     uint64_t data_bytes;     /* device CSR ring; 0 = 4 GiB */
     uint64_t staging_bytes;  /* pinned staging ring; 0 = 1 GiB */
     mbe_tuning tuning;       /* launch tuning of the serving kernel (0 fields =
-                                defaults; arena_words defaults to 1 << 20) */
+                                defaults: arena_words 1 << 20, lds_words 4096,
+                                waves_per_cu 4) */
 } mbe_server_config;
 
 typedef struct {
