@@ -112,6 +112,8 @@ def lib() -> C.CDLL:
                                      C.c_uint32, C.c_double, C.c_uint64, P(G)]
     L.mbe_gen_batch_graph.argtypes = [C.c_uint64, P(G)]
     L.mbe_solve.argtypes = [G, C.c_int, P(Tuning), P(Result)]
+    L.mbe_solve_list.argtypes = [G, C.c_int, P(Tuning), P(C.c_uint32), C.c_uint64,
+                                 P(C.c_uint64), P(Result)]
     L.mbe_batch_solve.argtypes = [P(G), C.c_size_t, C.c_int, P(Tuning), P(Result), P(Result)]
     L.mbe_batch_plan_create.argtypes = [P(G), C.c_size_t, C.c_int, P(Tuning), P(G)]
     L.mbe_batch_plan_run.argtypes = [G, P(Result), P(Result)]
@@ -127,7 +129,8 @@ def lib() -> C.CDLL:
     L.mbe_server_destroy.restype = None
     for name in ("mbe_graph_load_bi", "mbe_graph_write_bi", "mbe_graph_from_edge_list",
                  "mbe_graph_from_csr", "mbe_graph_info", "mbe_graph_csr", "mbe_gen_chung_lu",
-                 "mbe_gen_er_planted", "mbe_gen_batch_graph", "mbe_solve", "mbe_batch_solve",
+                 "mbe_gen_er_planted", "mbe_gen_batch_graph", "mbe_solve", "mbe_solve_list",
+                 "mbe_batch_solve",
                  "mbe_batch_plan_create", "mbe_batch_plan_run", "mbe_server_create",
                  "mbe_server_submit", "mbe_server_poll", "mbe_server_get_stats"):
         getattr(L, name).restype = C.c_int
@@ -246,6 +249,29 @@ def solve(graph: Graph, variant: str = "cuMBE", tuning: Optional[dict] = None,
     if not (allow_partial and r.partial):
         _check(st)
     return r
+
+
+def solve_list(graph: Graph, variant: str = "cuMBE", tuning: Optional[dict] = None):
+    """Every maximal biclique of `graph`: returns (Result, [(A, B), ...]) with A
+    the row ids and B the column ids of each biclique, as sorted tuples, and
+    the list sorted (mbe_solve_list: a size query, then the listing run)."""
+    L = lib()
+    t = _tuning(tuning)
+    tp = C.byref(t) if t else None
+    r = Result()
+    used = C.c_uint64(0)
+    _check(L.mbe_solve_list(graph.handle, VARIANTS[variant], tp, None, 0, C.byref(used), C.byref(r)))
+    n = used.value
+    buf = np.zeros(max(n, 1), dtype=np.uint32)
+    _check(L.mbe_solve_list(graph.handle, VARIANTS[variant], tp,
+                            buf.ctypes.data_as(C.POINTER(C.c_uint32)), n, C.byref(used), C.byref(r)))
+    out, i = [], 0
+    words = buf[:n].tolist()
+    while i < n:
+        a, b = words[i], words[i + 1]
+        out.append((tuple(words[i + 2:i + 2 + a]), tuple(words[i + 2 + a:i + 2 + a + b])))
+        i += 2 + a + b
+    return r, out
 
 
 def last_worker_busy() -> List[int]:

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <iterator>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -12,6 +13,7 @@
 #include <vector>
 #include <unistd.h>
 
+#include "hash.h"
 #include "host.h"
 
 namespace {
@@ -257,10 +259,18 @@ struct Pass0 {
     uint32_t workers = 0;
 };
 
+// Device side of a materialising solve (kernel records, see KParams::emit).
+struct Emit {
+    unsigned long long* pos = nullptr;  // 2 words, zeroed before the first pass
+    uint64_t* buf = nullptr;
+    uint64_t cap = 0;
+};
+
 // Run every root task of the uploaded graphs; roots whose local problem does
 // not fit a worker arena are retried with fewer workers and larger arenas.
 mbe_status run_tasks(Engine& E, Bufs& B, const Meta& m, int variant, const Tune& tu,
-                     std::vector<uint64_t>& res, mbe_result& out, const Pass0* pre = nullptr) {
+                     std::vector<uint64_t>& res, mbe_result& out, const Pass0* pre = nullptr,
+                     const Emit* em = nullptr) {
     {
         mbe_status s = ensure_lds(tu.lds_words);
         if (s != MBE_OK) return s;
@@ -293,6 +303,11 @@ mbe_status run_tasks(Engine& E, Bufs& B, const Meta& m, int variant, const Tune&
             mbe_status s = setup_pass(B, E.stream, E.abort_flag, m, tu, variant, task_list, n_tasks,
                                       arena_words, nw, kp);
             if (s != MBE_OK) return s;
+            if (em) {
+                kp.emit_pos = em->pos;
+                kp.emit = em->buf;
+                kp.emit_cap = em->cap;
+            }
             HIPC(hipEventRecord(E.ev0, E.stream));
             HIPC(enqueue_pass(kp, nw, pass == 0 ? m.ctl_words() : 2, E.stream));
         }
@@ -368,6 +383,72 @@ void fill_results(const std::vector<uint64_t>& res, size_t n, mbe_result& agg,
         }
 }
 
+// Turn the kernel's records into the caller's: L ids from the hash terms,
+// R' = the common neighbourhood of L' (checked against the reported size and
+// hash sum), file coordinates, records sorted. count is the solve's count.
+mbe_status decode_bicliques(const mbe::Prepared& p, const std::vector<uint64_t>& dev,
+                            uint64_t count, uint32_t* out, uint64_t words) {
+    const uint64_t salt_l = p.r_is_u ? mbe::kSaltV : mbe::kSaltU;
+    std::vector<uint32_t> recs;
+    std::vector<uint64_t> starts;
+    std::vector<uint32_t> a, b, t, bid;
+    recs.reserve(words);
+    for (size_t at = 0; at < dev.size();) {
+        const uint64_t nl = dev[at] & 0xffffffffu, nr = dev[at] >> 32, sr = dev[at + 1];
+        if (nl == 0 || at + 2 + nl > dev.size()) return fail(MBE_EINTERNAL, "malformed biclique record");
+        a.clear();
+        for (uint64_t i = 0; i < nl; i++) {
+            const uint64_t term = dev[at + 2 + i], id = mbe::unmix64(term) ^ salt_l;
+            if (id >= p.nL || p.hL[id] != term) return fail(MBE_EINTERNAL, "unknown L hash term");
+            a.push_back((uint32_t)id);
+        }
+        at += 2 + nl;
+        std::sort(a.begin(), a.end());
+        b.assign(p.adjL.begin() + p.offL[a[0]], p.adjL.begin() + p.offL[a[0] + 1]);
+        for (size_t i = 1; i < a.size() && !b.empty(); i++) {
+            t.clear();
+            std::set_intersection(b.begin(), b.end(), p.adjL.begin() + p.offL[a[i]],
+                                  p.adjL.begin() + p.offL[a[i] + 1], std::back_inserter(t));
+            b.swap(t);
+        }
+        uint64_t h = 0;
+        for (uint32_t r : b) h += p.hR[r];
+        if (b.size() != nr || h != sr)
+            return fail(MBE_EINTERNAL, "biclique's R side differs from the common neighbourhood");
+        bid.clear();
+        for (uint32_t r : b) bid.push_back(p.rank2id[r]);
+        std::sort(bid.begin(), bid.end());
+        const std::vector<uint32_t>& su = p.r_is_u ? bid : a;
+        const std::vector<uint32_t>& sv = p.r_is_u ? a : bid;
+        starts.push_back(recs.size());
+        recs.push_back((uint32_t)su.size());
+        recs.push_back((uint32_t)sv.size());
+        recs.insert(recs.end(), su.begin(), su.end());
+        recs.insert(recs.end(), sv.begin(), sv.end());
+    }
+    if (starts.size() != count || recs.size() != words)
+        return fail(MBE_EINTERNAL, "biclique records do not match the count");
+    auto key = [&](uint64_t s, int side) {
+        const uint32_t* r = recs.data() + s;
+        const uint32_t* first = r + 2 + (side ? r[0] : 0);
+        return std::make_pair(first, first + r[side]);
+    };
+    std::sort(starts.begin(), starts.end(), [&](uint64_t x, uint64_t y) {
+        auto ux = key(x, 0), uy = key(y, 0);
+        if (!std::equal(ux.first, ux.second, uy.first, uy.second))
+            return std::lexicographical_compare(ux.first, ux.second, uy.first, uy.second);
+        auto vx = key(x, 1), vy = key(y, 1);
+        return std::lexicographical_compare(vx.first, vx.second, vy.first, vy.second);
+    });
+    uint64_t o = 0;
+    for (uint64_t s : starts) {
+        const uint64_t len = 2 + (uint64_t)recs[s] + recs[s + 1];
+        memcpy(out + o, recs.data() + s, len * 4);
+        o += len;
+    }
+    return MBE_OK;
+}
+
 mbe_status ensure_lds(uint32_t lds_words) {
     static std::mutex mu;
     static uint32_t set = 0;
@@ -414,6 +495,8 @@ const char* mbe_status_str(mbe_status s) {
         case MBE_EIO: return "I/O error";
         case MBE_EBUSY: return "busy";
         case MBE_ENOTREADY: return "not ready";
+        case MBE_ERANGE: return "output buffer too small";
+        case MBE_EINTERNAL: return "internal self-check failed";
     }
     return "unknown status";
 }
@@ -540,6 +623,60 @@ mbe_status mbe_batch_solve(const mbe_graph* const* graphs, size_t n, mbe_variant
     fill_results(res, n, agg, per_graph);
     if (total) *total = agg;
     return s;
+}
+
+mbe_status mbe_solve_list(const mbe_graph* g, mbe_variant variant, const mbe_tuning* tuning,
+                          uint32_t* out, uint64_t cap_words, uint64_t* used_words,
+                          mbe_result* result) {
+    if (!g || !used_words || (unsigned)variant > 3u) return fail(MBE_EINVAL, "bad argument");
+    if (!out && cap_words) return fail(MBE_EINVAL, "null output with a nonzero capacity");
+    const double t0 = now_ms();
+    Engine& E = engine();
+    std::lock_guard<std::mutex> lock(E.mu);
+    mbe_status s = E.ensure_init();
+    if (s != MBE_OK) return s;
+    Tune tu = resolve(tuning, E.cus, E.lds_per_cu);
+    if (tu.side < -1 || tu.side > 1) return fail(MBE_EINVAL, "bad side");
+    mbe::Prepared p;
+    mbe::prepare(g->g, tu.side, p);
+    std::vector<const mbe::Prepared*> pp{&p};
+    std::vector<mbe::DevGraph> dg;
+    uint64_t total_tasks = 0;
+    s = upload(E.b, E.stream, pp, dg, total_tasks);
+    if (s != MBE_OK) return s;
+    // A record takes 2 + |L'| u64 words on the device and 2 + |L'| + |R'|
+    // (|R'| >= 1) u32 words for the caller, so cap_words device words hold
+    // every record whenever the caller's buffer holds them all.
+    struct Owned {
+        DevBuf b;
+        ~Owned() {
+            if (b.p) (void)hipFree(b.p);
+        }
+    } eb;
+    HIPC(eb.b.ensure((2 + cap_words) * 8));
+    HIPC(hipMemsetAsync(eb.b.p, 0, 16, E.stream));
+    const Emit em{eb.b.as<unsigned long long>(), eb.b.as<uint64_t>() + 2, cap_words};
+    mbe_result agg{};
+    std::vector<uint64_t> res;
+    s = run_tasks(E, E.b, meta_of(pp, total_tasks), (int)variant, tu, res, agg, nullptr, &em);
+    if (s != MBE_OK) return s;  // a watchdog stop leaves the list incomplete
+    unsigned long long pos[2] = {0, 0};
+    HIPC(hipMemcpy(pos, eb.b.p, 16, hipMemcpyDeviceToHost));
+    *used_words = pos[1];
+    fill_results(res, 1, agg, nullptr);
+    if (out) {
+        if (pos[1] > cap_words)
+            return fail(MBE_ERANGE, "output holds " + std::to_string(cap_words) +
+                                        " words; the bicliques take " + std::to_string(pos[1]));
+        std::vector<uint64_t> dev(pos[0]);
+        if (pos[0])
+            HIPC(hipMemcpy(dev.data(), em.buf, pos[0] * 8, hipMemcpyDeviceToHost));
+        s = decode_bicliques(p, dev, agg.count, out, pos[1]);
+        if (s != MBE_OK) return s;
+    }
+    agg.total_ms = now_ms() - t0;
+    if (result) *result = agg;
+    return MBE_OK;
 }
 
 mbe_status mbe_solve(const mbe_graph* g, mbe_variant variant, const mbe_tuning* tuning,

@@ -136,6 +136,11 @@ struct KParams {
     const uint64_t* staging;              // host-mapped staging buffer
     uint64_t* data;                       // device data buffer
     const unsigned long long* close;      // host-mapped: no more requests
+    // biclique materialisation (emit_pos != null): [0] u64 words reserved in
+    // emit, [1] u32 words the records take on the host
+    unsigned long long* emit_pos;
+    uint64_t* emit;
+    uint64_t emit_cap;                    // u64 words
 };
 
 }  // namespace mbe

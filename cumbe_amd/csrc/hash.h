@@ -32,6 +32,22 @@ MBE_HD uint64_t mix64(uint64_t z) {
     return z ^ (z >> 31);
 }
 
+// Inverse of mix64 (each step of the finaliser is invertible: x ^ (x >> s)
+// by repeated shifts, the odd multipliers by their inverses mod 2^64, found
+// by Newton's iteration y <- y (2 - c y), which doubles the correct low bits).
+constexpr uint64_t inv_odd64(uint64_t c) {
+    uint64_t y = c;  // correct to 3 bits for odd c
+    for (int i = 0; i < 5; i++) y *= 2 - c * y;
+    return y;
+}
+inline uint64_t unmix64(uint64_t z) {
+    z = z ^ (z >> 31) ^ (z >> 62);
+    z *= inv_odd64(0x94d049bb133111ebULL);
+    z = z ^ (z >> 27) ^ (z >> 54);
+    z *= inv_odd64(0xbf58476d1ce4e5b9ULL);
+    return z ^ (z >> 30) ^ (z >> 60);
+}
+
 MBE_HD uint64_t rotl64(uint64_t x, int r) { return (x << r) | (x >> (64 - r)); }
 
 // h(A, B) from the side sums and sizes.

@@ -7,6 +7,9 @@
 //   mbe gen chung-lu <out.bi> n_u n_v m gamma_u gamma_v max_deg seed
 //   mbe gen er-planted <out.bi> n_u n_v p n_planted lo hi top_frac seed
 //   mbe gen batch <out.bi> seed
+// and one debugging tool:
+//   mbe list <dataset.bi> <out.txt> [cuMBE|noRS|noES|noWS]
+//       writes every maximal biclique, one "A: rows | B: columns" line each
 #include <sys/stat.h>
 
 #include <cstdio>
@@ -67,6 +70,42 @@ static int cmd_gen(int argc, char** argv) {
     return s == MBE_OK ? 0 : die(s, "write");
 }
 
+static int cmd_list(int argc, char** argv) {
+    if (argc < 4) {
+        std::cerr << "usage: mbe list <dataset.bi> <out.txt> [cuMBE|noRS|noES|noWS]\n";
+        return 2;
+    }
+    std::string algo = argc > 4 ? argv[4] : "cuMBE";
+    mbe_variant var = algo == "noRS" ? MBE_NORS : algo == "noES" ? MBE_NOES : algo == "noWS" ? MBE_NOWS : MBE_CUMBE;
+    mbe_graph* g = nullptr;
+    mbe_status s = mbe_graph_load_bi(argv[2], &g);
+    if (s != MBE_OK) return die(s, "load");
+    uint64_t words = 0;
+    mbe_result r{};
+    s = mbe_solve_list(g, var, nullptr, nullptr, 0, &words, &r);  // size query
+    std::vector<uint32_t> buf(words);
+    if (s == MBE_OK) s = mbe_solve_list(g, var, nullptr, buf.data(), words, &words, &r);
+    mbe_graph_free(g);
+    if (s != MBE_OK) return die(s, "list");
+    std::ofstream out(argv[3]);
+    for (uint64_t i = 0; i < words;) {
+        const uint32_t na = buf[i], nb = buf[i + 1];
+        out << "A:";
+        for (uint32_t k = 0; k < na; k++) out << ' ' << buf[i + 2 + k];
+        out << " | B:";
+        for (uint32_t k = 0; k < nb; k++) out << ' ' << buf[i + 2 + na + k];
+        out << '\n';
+        i += 2 + (uint64_t)na + nb;
+    }
+    if (!out) {
+        std::cerr << "list: cannot write " << argv[3] << "\n";
+        return 1;
+    }
+    std::cout << "maximal bicliques: " << r.count << "\nhash: 0x" << std::hex << std::setw(16)
+              << std::setfill('0') << r.hash << std::dec << "\n";
+    return 0;
+}
+
 int main(int argc, char** argv) {
     if (argc < 2) {
         std::cerr << "usage: mbe <dataset.bi> [cuMBE|noRS|noES|noWS]\n";
@@ -75,6 +114,7 @@ int main(int argc, char** argv) {
     std::string a1 = argv[1];
     if (a1 == "convert") return cmd_convert(argc, argv);
     if (a1 == "gen") return cmd_gen(argc, argv);
+    if (a1 == "list") return cmd_list(argc, argv);
 
     time_t now = time(nullptr);
     std::string dataset = a1.substr(a1.rfind('/') + 1);

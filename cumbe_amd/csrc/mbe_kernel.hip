@@ -300,6 +300,30 @@ __device__ __forceinline__ void record(const DevGraph& G, u64 sl, u64 sr, u64 nl
     hs += h;
 }
 
+// Materialisation: one record per reported biclique, [nl | nr << 32, R hash
+// sum, the nl L' hash terms]. The terms identify the L ids (mix64 is a
+// bijection), R' = N(L') follows from the graph and is checked against the
+// sum and size. lbits: the L' bit list, or null when L' is every L bit.
+// Records past cap are counted, not written. Kept out of line: it runs only
+// in materialising launches and must not disturb the node loop's registers.
+__device__ __noinline__ void emit_biclique(unsigned long long* pos, uint64_t* out, uint64_t cap,
+                                           const u64* hl, const uint32_t* lbits, uint32_t nl,
+                                           u64 nr, u64 sr) {
+    u64 at = 0;
+    if (lane() == 0) {
+        at = atomicAdd(&pos[0], (u64)nl + 2);
+        atomicAdd(&pos[1], (u64)nl + nr + 2);
+    }
+    at = uni64(at);
+    if (at + nl + 2 > cap) return;
+    uint64_t* o = out + at;
+    if (lane() == 0) {
+        o[0] = (u64)nl | (nr << 32);
+        o[1] = sr;
+    }
+    for (uint32_t l = lane(); l < nl; l += 64) o[2 + l] = hl[lbits ? lbits[l] : l];
+}
+
 // Candidate selection at a level >= 1 (src/mbe_cuMBE.cuh:345-390): scan P in
 // descending candidate order (words high to low, one lane per set bit) and
 // keep the first strict minimum of |row(v) & L|; with ES stop as soon as the
@@ -1105,6 +1129,7 @@ __device__ void dfs(const KParams& kp, const DevGraph& G, uint32_t g, Local& S, 
             const u64 sr = F[1] + S.hc[xk] + o.abh;
             const u64 nr = F[2] + 1 + o.ab;
             record(G, sl, sr, nl, nr, cnt, hs);
+            if (kp.emit_pos) emit_biclique(kp.emit_pos, kp.emit, kp.emit_cap, S.hl, S.lbits, nl, nr, sr);
             sec.mark(kSecReport);
             if (o.descend) {
                 if (lane() == 0) {
@@ -1301,6 +1326,7 @@ __device__ int build_root(const KParams& kp, const DevGraph& G, uint32_t x, u64*
     sl = red_add64(sl);
     const u64 sr = G.hR[x] + abh, nr = 1 + ab;
     record(G, sl, sr, S.d, nr, cnt, hs);
+    if (kp.emit_pos) emit_biclique(kp.emit_pos, kp.emit, kp.emit_cap, S.hl, nullptr, S.d, nr, sr);
     if (S.kx == ab) return 0;  // P' empty
     if (lane() == 0) {
         F1[1] = sr;

@@ -37,7 +37,9 @@ typedef enum {
     MBE_EOVERBUDGET = -5,  /* request exceeds the admission budget */
     MBE_EIO = -6,          /* file could not be read or written */
     MBE_EBUSY = -7,        /* server queue full; retry later */
-    MBE_ENOTREADY = -8     /* poll: request not finished yet */
+    MBE_ENOTREADY = -8,    /* poll: request not finished yet */
+    MBE_ERANGE = -9,       /* output buffer too small; the size needed is returned */
+    MBE_EINTERNAL = -10    /* the engine's results failed a self-check */
 } mbe_status;
 
 /* The four algorithm variants of the reference (src/main.cu:121-122). */
@@ -145,6 +147,22 @@ mbe_status mbe_gen_batch_graph(uint64_t seed, mbe_graph** out);
 /* One graph, one stream, synchronous. tuning may be NULL. */
 mbe_status mbe_solve(const mbe_graph* g, mbe_variant variant, const mbe_tuning* tuning,
                      mbe_result* out);
+
+/* Materialised bicliques. Enumerates like mbe_solve and returns every
+ * maximal biclique (A, B), A a set of rows (U) and B of columns (V) of the
+ * graph, as records of uint32 words: |A|, |B|, the ids of A ascending, the
+ * ids of B ascending. Records are sorted (by A's ids, then B's), so a list
+ * does not depend on the variant, the tuning or the run. *used_words receives
+ * the words all records take. With out == NULL (and cap_words 0) only that
+ * size is computed; with cap_words < *used_words nothing is written and
+ * MBE_ERANGE is returned. Each record is checked on the host: B must be the
+ * common neighbourhood of A, with the size and hash the kernel reported
+ * (MBE_EINTERNAL otherwise). The device holds cap_words u64 words while it
+ * runs. result may be NULL. No reference counterpart: the reference only
+ * counts (src/mbe_cuMBE.cuh:295-300). */
+mbe_status mbe_solve_list(const mbe_graph* g, mbe_variant variant, const mbe_tuning* tuning,
+                          uint32_t* out, uint64_t cap_words, uint64_t* used_words,
+                          mbe_result* result);
 
 /* Many graphs in one persistent launch sharing one task pool; per_graph[i]
  * receives graph i's count and hash (timing fields are the batch's).

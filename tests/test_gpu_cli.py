@@ -72,3 +72,22 @@ def test_variance_flag_through_the_abi(c0):
     assert len(busy) == 128 and max(busy) > 0
     assert (r.count, f"{r.hash:#018x}") == (gold["count"], gold["hash"])
     assert sum(busy) <= sum(r.sections)
+
+
+def test_cli_list(c0):
+    d, path, gold = c0
+    out = run(d, ["list", str(path), str(d / "c0.txt"), "noES"])
+    assert out == [f"maximal bicliques: {gold['count']}", f"hash: {gold['hash']}"]
+    lines = (d / "c0.txt").read_text().splitlines()
+    assert len(lines) == gold["count"]
+    a, b = lines[0].split(" | ")
+    assert a.startswith("A:") and b.startswith("B:")
+    # each line is a biclique of the file: every A row is adjacent to every B column
+    g = M.Graph.load_bi(str(path))
+    n_u, n_v, off, adj = g.csr()
+    for line in lines[:200]:
+        a, b = line.split(" | ")
+        rows = [int(x) for x in a.split()[1:]]
+        cols = set(int(x) for x in b.split()[1:])
+        for u in rows:
+            assert cols <= set(adj[off[u]:off[u + 1]].tolist())

@@ -6,6 +6,7 @@ The .bi format follows the reference reader and writer (src/main.cu:38-48,
 data/gen_bi.cpp:72-79): `NUM_R NUM_L NUM_EDGES`, a blank line, NUM_R lines of
 `start length`, then NUM_EDGES entries of `col 1`.
 """
+import ctypes as C
 import json
 import os
 import re
@@ -38,11 +39,18 @@ def test_library_exports_every_declared_function():
 
 def test_status_strings_and_errors_without_device():
     lib = M.lib()
-    for code in range(0, -9, -1):
-        assert lib.mbe_status_str(code)
+    for code in range(0, -11, -1):
+        assert lib.mbe_status_str(code) != b"unknown status"
     with pytest.raises(M.MbeError) as e:
         M.Graph.from_csr(2, 2, [0, 1, 2], [0, 5])  # id 5 out of range
     assert e.value.status == -1
+    # argument checks of the listing entry point come before any device call
+    used = C.c_uint64(0)
+    assert lib.mbe_solve_list(None, 0, None, None, 0, C.byref(used), None) == -1
+    g = M.Graph.from_csr(2, 2, [0, 1, 2], [0, 1])
+    assert lib.mbe_solve_list(g.handle, 0, None, None, 5, C.byref(used), None) == -1
+    assert lib.mbe_solve_list(g.handle, 7, None, None, 0, C.byref(used), None) == -1
+    assert lib.mbe_solve_list(g.handle, 0, None, None, 0, None, None) == -1
 
 
 def test_bi_round_trip(tmp_path):
