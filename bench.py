@@ -41,6 +41,8 @@ import re
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -169,7 +171,7 @@ def cpu_baseline(g, gpu_tuning):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--variant", default="cuMBE", choices=list(M.VARIANTS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -208,9 +210,12 @@ def main():
     kernel_ms = wall_ms = 0.0
     launches = 0
     first = None
+    step_ms, e2e_ms = [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         r = M.solve(g, args.variant, tun)
+        step_ms.append(r.kernel_ms)
+        e2e_ms.append(r.total_ms)
         first = first or (r.count, r.hash)
         if (r.count, r.hash) != first:
             raise SystemExit("non-deterministic result across steps")
@@ -241,6 +246,11 @@ def main():
         "warmup": args.warmup,
         "ms_per_step": kernel_ms / args.steps,
         "wall_ms_per_step": wall_ms / args.steps,
+        # kernel time of each step, its median, and the host's end-to-end time
+        # per solve (preparation, upload, launch, readback)
+        "step_ms": step_ms,
+        "median_ms_per_step": float(np.median(step_ms)),
+        "end_to_end_ms_per_step": float(np.mean(e2e_ms)),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,

@@ -630,6 +630,7 @@ mbe_status mbe_solve_list(const mbe_graph* g, mbe_variant variant, const mbe_tun
                           mbe_result* result) {
     if (!g || !used_words || (unsigned)variant > 3u) return fail(MBE_EINVAL, "bad argument");
     if (!out && cap_words) return fail(MBE_EINVAL, "null output with a nonzero capacity");
+    if (cap_words > ((uint64_t)1 << 40)) return fail(MBE_ENOMEM, "output capacity above 2^40 words");
     const double t0 = now_ms();
     Engine& E = engine();
     std::lock_guard<std::mutex> lock(E.mu);

@@ -158,7 +158,7 @@ mbe_status mbe_solve(const mbe_graph* g, mbe_variant variant, const mbe_tuning* 
  * MBE_ERANGE is returned. Each record is checked on the host: B must be the
  * common neighbourhood of A, with the size and hash the kernel reported
  * (MBE_EINTERNAL otherwise). The device holds cap_words u64 words while it
- * runs. result may be NULL. No reference counterpart: the reference only
+ * runs (cap_words at most 2^40, else MBE_ENOMEM). result may be NULL. No reference counterpart: the reference only
  * counts (src/mbe_cuMBE.cuh:295-300). */
 mbe_status mbe_solve_list(const mbe_graph* g, mbe_variant variant, const mbe_tuning* tuning,
                           uint32_t* out, uint64_t cap_words, uint64_t* used_words,
