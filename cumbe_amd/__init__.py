@@ -81,6 +81,7 @@ SECTION_NAMES = ["other", "root_build", "select", "lprime", "check", "expand", "
                  "compact", "donate", "fetch", "wait"]
 FLAG_SECTIONS = 1
 FLAG_VARIANCE = 2
+FLAG_REFERENCE_TREE = 4   # walk exactly the reference tree (no twin pruning)
 
 
 _lib = None

@@ -101,6 +101,7 @@ struct KParams {
     uint32_t lds_words;         // dynamic LDS per worker (u64 words)
     uint32_t list_cutoff;       // |P|+|Q| at or below which a node tests rows (list form)
     uint32_t compact_ratio;     // re-localise a child whose universe shrank this much (0 = never)
+    uint32_t twins;             // 1: drop twin candidates of the selected vertex from P
     // roots that did not fit the arena
     unsigned long long* overflow_count;
     uint64_t* overflow_list;

@@ -667,6 +667,83 @@ __device__ NodeOut process_fwd(const Local& S, u64* F, u64* Fc, uint32_t nl) {
     return o;
 }
 
+// Twin pruning (the candidate removal of iMBEA, Zhang et al., BMC
+// Bioinformatics 2014, which the reference leaves out). A candidate v left in
+// P with N(v) n L == L' = N(x) n L would, selected later at this level, give
+// the same L', which x (by then in Q) covers: a non-maximal leaf. A node
+// below a later sibling that holds v in R has its L inside L', which x covers
+// too, so v is no witness that x is not. Such v leave P and enter neither Q
+// nor X: the tree loses exactly these leaves and reports the same bicliques.
+// T = P & AND of the L' columns holds the candidates containing L'; its
+// twins have no neighbour in L \ L'. The scratch list (free until the next
+// compaction) holds T.
+// Kept out of line, with plain arguments, so that the node loop's inlining
+// and registers stay as they are; returns |T| << 32 | removed.
+__device__ __noinline__ u64 prune_twins(u64* F, const u64* Lc, const u64* cols, const u64* rows,
+                                        const uint32_t* lbits, uint32_t* list, uint32_t W,
+                                        uint32_t Wc, uint32_t nl) {
+    const u64* L = F + kFrameHdr;
+    u64* P = F + kFrameHdr + W;
+    uint32_t nt = 0;
+    for (uint32_t w0 = 0; w0 < Wc; w0 += 64) {
+        const uint32_t w = w0 + lane();
+        u64 a = w < Wc ? P[w] : 0;
+        if (!__ballot(a != 0)) continue;
+        uint32_t l = 0;
+        for (; l + 4 <= nl && a; l += 4) {
+            const uint32_t j0 = lbits[l], j1 = lbits[l + 1];
+            const uint32_t j2 = lbits[l + 2], j3 = lbits[l + 3];
+            a &= cols[(size_t)j0 * Wc + w] & cols[(size_t)j1 * Wc + w] &
+                 cols[(size_t)j2 * Wc + w] & cols[(size_t)j3 * Wc + w];
+        }
+        for (; l < nl && a; l++) a &= cols[(size_t)lbits[l] * Wc + w];
+        const uint32_t c = __popcll(a);
+        const uint32_t inc = scan_add32(c);
+        uint32_t k = nt + inc - c;
+        while (a) {
+            const uint32_t b = __builtin_ctzll(a);
+            a &= a - 1;
+            list[k++] = w * 64 + b;
+        }
+        nt += rdlane32(inc, 63);
+    }
+    if (nt == 0) return 0;
+    wsync();
+    uint32_t removed = 0;
+    if (W <= 64) {
+        // lane = group * Wp + word: 64 / Wp candidates per pass
+        const uint32_t lg = W > 1 ? 32 - __builtin_clz(W - 1) : 0;
+        const uint32_t Wp = 1u << lg;
+        const uint32_t G = 64u >> lg;
+        const uint32_t w = lane() & (Wp - 1), gi = lane() >> lg;
+        const u64 m = w < W ? (L[w] & ~Lc[w]) : 0;
+        const u64 gmask = (Wp == 64 ? ~0ull : ((1ull << Wp) - 1)) << (gi * Wp);
+        for (uint32_t b0 = 0; b0 < nt; b0 += G) {
+            const uint32_t i = b0 + gi;
+            const uint32_t c = i < nt ? list[i] : 0;
+            const bool hit = i < nt && w < W && (rows[(size_t)c * W + w] & m) != 0;
+            const u64 bad = __ballot(hit);
+            const bool twin = i < nt && w == 0 && (bad & gmask) == 0;
+            if (twin) atomicAnd(&P[c >> 6], ~(1ull << (c & 63)));
+            removed += __popcll(__ballot(twin));
+        }
+    } else {
+        for (uint32_t i = 0; i < nt; i++) {
+            const uint32_t c = list[i];
+            bool hit = false;
+            for (uint32_t w = lane(); w < W; w += 64)
+                hit |= (rows[(size_t)c * W + w] & L[w] & ~Lc[w]) != 0;
+            if (!__ballot(hit)) {
+                if (lane() == 0) P[c >> 6] &= ~(1ull << (c & 63));
+                removed++;
+            }
+        }
+    }
+    if (lane() == 0 && removed) F[3] -= removed;  // |P| is the low half
+    wsync();
+    return ((u64)nt << 32) | removed;
+}
+
 // ------------------------------------------------------------------ deque
 // Donated subtrees travel through a bounded multi-producer multi-consumer
 // ring of fixed-size slots in HBM (per-slot sequence numbers, Vyukov's
@@ -1123,6 +1200,11 @@ __device__ void dfs(const KParams& kp, const DevGraph& G, uint32_t g, Local& S, 
         // and L, the L' columns (or the P and Q rows) with P/Q/X, L' terms
         words += (u64)pc * S.W + S.Wc + 2 * S.W + nl +
                  (fwd ? (u64)(pc - 1 + qc) * S.W + 3 * S.Wc : (u64)S.Wc * (nl + 3));
+        if (kp.twins && pc > 1) {
+            const u64 r = prune_twins(F, Fc + kFrameHdr, S.cols, S.rows, S.lbits, S.list, S.W,
+                                      S.Wc, nl);
+            words += (u64)S.Wc * (nl + 1) + (r >> 32) * S.W;  // P and L' columns, T's rows
+        }
         sec.mark(o.maximal ? kSecExpand : kSecCheck);
         if (o.maximal) {
             const u64 sl = sum_l(S, nl);

@@ -109,6 +109,13 @@ typedef struct {
  * reference's DEVARIANCE build (src/mbe_noES.cuh:49-55); read it with
  * mbe_last_worker_busy. Implies MBE_FLAG_SECTIONS. */
 #define MBE_FLAG_VARIANCE 2u
+/* Walk exactly the reference's search tree. By default a candidate whose
+ * neighbourhood within the current L equals that of the vertex just
+ * selected is dropped from P (iMBEA's rule, which the reference leaves
+ * out): each such candidate would only yield a non-maximal leaf. The
+ * bicliques reported are the same either way; the tree and the time are
+ * not. */
+#define MBE_FLAG_REFERENCE_TREE 4u
 
 typedef struct mbe_batch_plan mbe_batch_plan;  /* resident batch + captured launch */
 

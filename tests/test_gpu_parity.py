@@ -128,3 +128,19 @@ def test_watchdog_stops_cleanly():
     ref = O.solve(*G.crown(10))
     r = M.solve(gpu_graph(G.crown(10)), "cuMBE")
     assert not r.partial and (r.count, r.hash) == (ref.count, ref.hash)
+
+
+def test_reference_tree_flag_same_results_more_nodes():
+    """Twin pruning (default) and the reference's own tree (FLAG_REFERENCE_TREE)
+    report the same bicliques; the reference tree visits at least as many
+    nodes."""
+    g = G.random_graph(150, 120, 0.12, 9090)
+    ref = O.solve(*g)
+    gg = gpu_graph(g)
+    for v in VARIANTS:
+        say(f"reference tree variant {v}")
+        a = M.solve(gg, v)
+        b = M.solve(gg, v, {"flags": M.FLAG_REFERENCE_TREE, "side": -1})
+        assert (a.count, a.hash) == (ref.count, ref.hash), v
+        assert (b.count, b.hash) == (ref.count, ref.hash), v
+        assert b.nodes >= a.nodes, (v, a.nodes, b.nodes)
