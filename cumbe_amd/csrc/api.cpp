@@ -223,7 +223,7 @@ mbe_status setup_pass(Bufs& B, hipStream_t stream, unsigned long long* abort_hos
     kp.lds_words = tu.lds_words;
     kp.list_cutoff = tu.list_cutoff;
     kp.compact_ratio = tu.compact_ratio;
-    kp.twins = (tu.flags & MBE_FLAG_REFERENCE_TREE) ? 0 : 1;
+    kp.twins = (tu.flags & MBE_FLAG_REFERENCE_TREE) ? 0 : 2;
     HIPC(hipHostGetDevicePointer((void**)&kp.abort, abort_host, 0));
     kp.node_count = kp.stats + mbe::kStatNodeBudget;
     kp.node_budget = tu.max_nodes;

@@ -428,6 +428,8 @@ struct NodeOut {
     uint32_t pn;    // |P'|
     uint32_t qn;    // |Q'|
     uint32_t un;    // |P' u Q'|
+    u64 tw;         // Wc <= 64 reverse scan: lane w < Wc holds word w of P & A
+    bool has_tw;
 };
 
 // L' = row(x) & L into the child frame and its set-bit list; returns |L'|.
@@ -464,7 +466,7 @@ __device__ NodeOut process_rs(const Local& S, u64* F, u64* Fc, uint32_t nl) {
     u64* Pc = Fc + kFrameHdr + S.W;
     u64* Qc = Pc + S.Wc;
     u64* Xc = Qc + S.Wc;
-    NodeOut o{true, false, 0, 0, 0, 0, 0};
+    NodeOut o{true, false, 0, 0, 0, 0, 0, 0, false};
     u64 abh = 0;
     if (S.Wc <= 64) {
         // lane = group * Wp + word (Wp = Wc rounded up to a power of two):
@@ -496,6 +498,8 @@ __device__ NodeOut process_rs(const Local& S, u64* F, u64* Fc, uint32_t nl) {
         for (uint32_t m = Wp; m < 64; m <<= 1) a &= shfl_xor64(a, m);
         const bool mine = valid && gi == 0;
         const u64 q = mine ? Q[w] : 0;
+        o.tw = mine ? a & P[w] : 0;  // candidates of P containing L'
+        o.has_tw = true;
         const bool nonmax = mine && (a & q) != 0;
         o.maximal = __ballot(nonmax) == 0;
         if (o.maximal) {  // O is combined only where it is used
@@ -608,7 +612,7 @@ __device__ NodeOut process_fwd(const Local& S, u64* F, u64* Fc, uint32_t nl) {
     u64* Pc = Fc + kFrameHdr + S.W;
     u64* Qc = Pc + S.Wc;
     u64* Xc = Qc + S.Wc;
-    NodeOut o{true, false, 0, 0, 0, 0, 0};
+    NodeOut o{true, false, 0, 0, 0, 0, 0, 0, false};
     for (uint32_t w = 0; w < S.Wc; w++) {
         const u64 qw = uni64(Q[w]);
         u64 qn = 0;
@@ -679,17 +683,19 @@ __device__ NodeOut process_fwd(const Local& S, u64* F, u64* Fc, uint32_t nl) {
 // compaction) holds T.
 // Kept out of line, with plain arguments, so that the node loop's inlining
 // and registers stay as they are; returns |T| << 32 | removed.
+// tw: when has_tw, lane w < Wc already holds word w of T (the reverse scan
+// computed it), which saves the column loads.
 __device__ __noinline__ u64 prune_twins(u64* F, const u64* Lc, const u64* cols, const u64* rows,
                                         const uint32_t* lbits, uint32_t* list, uint32_t W,
-                                        uint32_t Wc, uint32_t nl) {
+                                        uint32_t Wc, uint32_t nl, u64 tw, bool has_tw) {
     const u64* L = F + kFrameHdr;
     u64* P = F + kFrameHdr + W;
     uint32_t nt = 0;
     for (uint32_t w0 = 0; w0 < Wc; w0 += 64) {
         const uint32_t w = w0 + lane();
-        u64 a = w < Wc ? P[w] : 0;
+        u64 a = has_tw ? tw : (w < Wc ? P[w] : 0);
         if (!__ballot(a != 0)) continue;
-        uint32_t l = 0;
+        uint32_t l = has_tw ? nl : 0;
         for (; l + 4 <= nl && a; l += 4) {
             const uint32_t j0 = lbits[l], j1 = lbits[l + 1];
             const uint32_t j2 = lbits[l + 2], j3 = lbits[l + 3];
@@ -1200,10 +1206,11 @@ __device__ void dfs(const KParams& kp, const DevGraph& G, uint32_t g, Local& S, 
         // and L, the L' columns (or the P and Q rows) with P/Q/X, L' terms
         words += (u64)pc * S.W + S.Wc + 2 * S.W + nl +
                  (fwd ? (u64)(pc - 1 + qc) * S.W + 3 * S.Wc : (u64)S.Wc * (nl + 3));
-        if (kp.twins && pc > 1) {
+        // twins: 1 = only where the reverse scan already gave T, 2 = everywhere
+        if (pc > 1 && (kp.twins > 1 || (kp.twins == 1 && o.has_tw))) {
             const u64 r = prune_twins(F, Fc + kFrameHdr, S.cols, S.rows, S.lbits, S.list, S.W,
-                                      S.Wc, nl);
-            words += (u64)S.Wc * (nl + 1) + (r >> 32) * S.W;  // P and L' columns, T's rows
+                                      S.Wc, nl, o.tw, o.has_tw);
+            words += (o.has_tw ? 0 : (u64)S.Wc * (nl + 1)) + (r >> 32) * S.W;
         }
         sec.mark(o.maximal ? kSecExpand : kSecCheck);
         if (o.maximal) {

@@ -144,6 +144,10 @@ struct mbe_server {
         mbe_status s = setup_pass(b, stream, abort_flag, m, tu, 0, {}, 0, tu.arena_words,
                                   tu.workers, kp);
         if (s != MBE_OK) return s;
+        // the serving kernel walks the reference tree: with twin pruning a
+        // large request served alone measured 575 ms against 310 ms without
+        // (the same graph solved alone: 50 ms against 149 ms)
+        kp.twins = 0;
         kp.graphs = gtable.as<mbe::DevGraph>();
         kp.n_graphs = 0;
         kp.remaining = rem.as<unsigned long long>();
