@@ -103,6 +103,7 @@ typedef struct {
     uint64_t *hR, *hL;             /* mix64(file id ^ salt of its side) */
     int r_is_u;                    /* 1 if the enumerated side is U (rows) */
     int rs, es;
+    int prune;                     /* twin removal (not the reference's tree) */
     mbe_oracle_emit_fn emit;
     void* user;
     atomic_llong cursor;           /* the reference's P_ptr, src/header.cuh:35 */
@@ -115,7 +116,7 @@ typedef struct {
     int32_t *v2P, *P, *v2Q, *Q, *R, *num_N_u;
     int32_t *x, *L_lp, *R_lp, *P_lp, *Q_lp, *pre_min;
     uint32_t *ea, *eb;             /* emit scratch */
-    uint64_t count, hash, work;
+    uint64_t count, hash, work, nodes;
     int err;
 } worker;
 
@@ -339,6 +340,29 @@ static void select_candidate(worker* w, int lvl) {
     w->v2P[tmp] = imin;
 }
 
+/* Twin removal of iMBEA (Zhang et al., BMC Bioinformatics 2014), which the
+ * reference leaves out; used only to check the GPU engine's default pruning.
+ * After node x at lvl, every v left in P with N(v) n L = L' = N(x) n L leaves
+ * P: selected later it would repeat L', which x (then in Q) covers. */
+static void drop_twins(worker* w, int lvl) {
+    const problem* pb = w->pb;
+    const int32_t Llp = w->L_lp[lvl], nl = w->L_lp[lvl + 1];
+    for (int32_t i = w->P_lp[lvl]; i-- > 0;) {
+        const int32_t v = w->P[i];
+        int32_t in_l = 0, in_lp = 0;
+        for (uint32_t e = pb->offR[v]; e < pb->offR[v + 1]; e++) {
+            const int32_t l = w->u2L[pb->adjR[e]];
+            if (l < Llp) { in_l++; if (l < nl) in_lp++; }
+        }
+        if (in_l == nl && in_lp == nl) {  /* swap v out of the P prefix */
+            const int32_t last = --w->P_lp[lvl];
+            const int32_t tmp = w->P[last];
+            w->P[last] = v; w->P[i] = tmp;
+            w->v2P[v] = last; w->v2P[tmp] = i;
+        }
+    }
+}
+
 /* Claim the next root (decreasing index). Roots outside the requested
  * subset are skipped; the Q sweep of the next claim moves them into Q
  * exactly as roots claimed by other blocks are (src/mbe_noWS.cuh:135-150). */
@@ -378,7 +402,10 @@ static void run_worker(worker* w, problem* pb) {
             while (w->P_lp[lvl] != 0) {
                 select_candidate(w, lvl);
                 w->x[lvl] = w->P[--w->P_lp[lvl]];
-                if (process(w, lvl)) { is_recursive = 1; break; }
+                w->nodes++;
+                int descend = process(w, lvl);
+                if (pb->prune) drop_twins(w, lvl);
+                if (descend) { is_recursive = 1; break; }
                 q_push(w, &w->Q_lp[lvl], w->x[lvl]);
                 is_up = w->P_lp[lvl] == 0;
             }
@@ -407,7 +434,7 @@ static double now_s(void) {
 static int solve_impl(uint32_t n_u, uint32_t n_v, const uint32_t* off_u, const uint32_t* adj_u,
                       int rs, int es, int side, int threads, mbe_oracle_emit_fn emit,
                       void* user, int64_t rb, int64_t re, int64_t rstride,
-                      mbe_oracle_result* out) {
+                      mbe_oracle_result* out, int prune, uint64_t* nodes) {
     if (!out || (!off_u && n_u) || threads < 1) return -1;
     memset(out, 0, sizeof(*out));
     if (emit && threads != 1) return -1;
@@ -425,7 +452,7 @@ static int solve_impl(uint32_t n_u, uint32_t n_v, const uint32_t* off_u, const u
     pb.offR = Rs->off; pb.adjR = Rs->adj;
     pb.offL = Ls->off; pb.adjL = Ls->adj;
     pb.r_is_u = r_is_u;
-    pb.rs = rs; pb.es = es; pb.emit = emit; pb.user = user;
+    pb.rs = rs; pb.es = es; pb.emit = emit; pb.user = user; pb.prune = prune;
     pb.sub_begin = rb;
     pb.sub_end = re > 0 ? re : (int64_t)1 << 62;
     pb.sub_stride = rstride > 0 ? rstride : 1;
@@ -467,6 +494,7 @@ static int solve_impl(uint32_t n_u, uint32_t n_v, const uint32_t* off_u, const u
             out->count += ws[t].count;
             out->hash += ws[t].hash;
             out->work += ws[t].work;
+            if (nodes) *nodes += ws[t].nodes;
         }
     }
     for (int t = 0; t < threads; t++) worker_free(&ws[t]);
@@ -479,7 +507,16 @@ done:
 
 int mbe_oracle_solve(uint32_t n_u, uint32_t n_v, const uint32_t* off_u, const uint32_t* adj_u,
                      int rs, int es, int side, int threads, mbe_oracle_result* out) {
-    return solve_impl(n_u, n_v, off_u, adj_u, rs, es, side, threads, NULL, NULL, 0, 0, 1, out);
+    return solve_impl(n_u, n_v, off_u, adj_u, rs, es, side, threads, NULL, NULL, 0, 0, 1, out,
+                      0, NULL);
+}
+
+int mbe_oracle_solve_twins(uint32_t n_u, uint32_t n_v, const uint32_t* off_u,
+                           const uint32_t* adj_u, int prune, mbe_oracle_result* out,
+                           uint64_t* nodes) {
+    if (nodes) *nodes = 0;
+    return solve_impl(n_u, n_v, off_u, adj_u, 1, 1, -1, 1, NULL, NULL, 0, 0, 1, out, prune,
+                      nodes);
 }
 
 int mbe_oracle_solve_roots(uint32_t n_u, uint32_t n_v, const uint32_t* off_u,
@@ -488,13 +525,14 @@ int mbe_oracle_solve_roots(uint32_t n_u, uint32_t n_v, const uint32_t* off_u,
                            mbe_oracle_result* out) {
     if (root_begin < 0 || root_stride < 0) return -1;
     return solve_impl(n_u, n_v, off_u, adj_u, rs, es, side, threads, NULL, NULL, root_begin,
-                      root_end, root_stride, out);
+                      root_end, root_stride, out, 0, NULL);
 }
 
 int mbe_oracle_solve_emit(uint32_t n_u, uint32_t n_v, const uint32_t* off_u,
                           const uint32_t* adj_u, int rs, int es, int side,
                           mbe_oracle_emit_fn emit, void* user, mbe_oracle_result* out) {
-    return solve_impl(n_u, n_v, off_u, adj_u, rs, es, side, 1, emit, user, 0, 0, 1, out);
+    return solve_impl(n_u, n_v, off_u, adj_u, rs, es, side, 1, emit, user, 0, 0, 1, out, 0,
+                      NULL);
 }
 
 /* ------------------------------------------------------------ brute force */

@@ -80,6 +80,13 @@ int mbe_oracle_solve_roots(uint32_t n_u, uint32_t n_v, const uint32_t* off_u,
                            int64_t root_begin, int64_t root_end, int64_t root_stride,
                            mbe_oracle_result* out);
 
+/* RS + ES with the reference's side choice, one thread; prune = 1 adds
+ * iMBEA's twin removal (not the reference's tree), which the GPU engine
+ * uses by default. *nodes receives the tree nodes below the roots. */
+int mbe_oracle_solve_twins(uint32_t n_u, uint32_t n_v, const uint32_t* off_u,
+                           const uint32_t* adj_u, int prune, mbe_oracle_result* out,
+                           uint64_t* nodes);
+
 int mbe_oracle_solve_emit(uint32_t n_u, uint32_t n_v, const uint32_t* off_u,
                           const uint32_t* adj_u, int rs, int es, int side,
                           mbe_oracle_emit_fn emit, void* user,

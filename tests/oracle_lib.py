@@ -42,6 +42,9 @@ def lib() -> C.CDLL:
                                              C.c_int, C.c_int, C.c_int64, C.c_int64, C.c_int64,
                                              C.POINTER(OracleResult)]
         L.mbe_oracle_solve_roots.restype = C.c_int
+        L.mbe_oracle_solve_twins.argtypes = [C.c_uint32, C.c_uint32, P, P, C.c_int,
+                                             C.POINTER(OracleResult), C.POINTER(C.c_uint64)]
+        L.mbe_oracle_solve_twins.restype = C.c_int
         L.mbe_oracle_brute.argtypes = [C.c_uint32, C.c_uint32, P, P, C.POINTER(OracleResult)]
         L.mbe_oracle_brute.restype = C.c_int
         L.mbe_oracle_biclique_hash.argtypes = [P, C.c_uint32, P, C.c_uint32]
@@ -92,6 +95,17 @@ def enumerate_sets(n_u, n_v, off, adj, rs=1, es=1, side=-1):
     if rc:
         raise ValueError(f"oracle rejected input ({rc})")
     return out, r
+
+
+def solve_twins(n_u, n_v, off, adj, prune):
+    """(result, tree nodes) with or without iMBEA's twin removal."""
+    off, adj, po, pa = _arrs(off, adj)
+    r = OracleResult()
+    nodes = C.c_uint64(0)
+    rc = lib().mbe_oracle_solve_twins(n_u, n_v, po, pa, prune, C.byref(r), C.byref(nodes))
+    if rc:
+        raise ValueError(f"oracle rejected input ({rc})")
+    return r, nodes.value
 
 
 def brute(n_u, n_v, off, adj) -> OracleResult:

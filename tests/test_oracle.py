@@ -11,6 +11,10 @@ the oracle is pinned by what the mathematics fixes:
     exactly the maximal bicliques);
   * invariances: enumerated-side choice, transpose (count), the four variant
     switches (rs/es), root-parallel threads.
+It also checks iMBEA's twin removal, which the GPU engine applies by default:
+the pruned tree reports the same bicliques as the reference's tree (and as
+brute force), and a wrong rule (dropping candidates that merely contain L')
+fails these cases.
 """
 import itertools
 
@@ -137,3 +141,38 @@ def test_duplicates_are_removed():
     r1 = O.solve(n_u, n_v, off, adj)
     r2 = O.solve(*dedup)
     assert (r1.count, r1.hash) == (r2.count, r2.hash)
+
+
+def twin_heavy(n_u, n_v, classes, p, seed):
+    """Random graph whose U rows repeat a few neighbourhoods (many twins),
+    plus some noise edges."""
+    rng = np.random.default_rng(seed)
+    base = rng.random((classes, n_v)) < p
+    rows = base[rng.integers(0, classes, n_u)] | (rng.random((n_u, n_v)) < 0.02)
+    return G.csr_from_edges(n_u, n_v, np.argwhere(rows))
+
+
+TWIN_CASES = [(f"suite_{n}", g) for n, g in SUITE] + [
+    (f"twins_{s}", twin_heavy(40 + 7 * s, 30 + 5 * s, 4 + s, 0.3, 300 + s)) for s in range(6)]
+
+
+@pytest.mark.parametrize("name,g", TWIN_CASES, ids=[n for n, _ in TWIN_CASES])
+def test_twin_removal_keeps_the_bicliques(name, g):
+    """iMBEA's twin removal (the GPU engine's default) reports the same count
+    and hash as the reference's tree, visiting no more nodes; on tiny graphs
+    both equal brute force."""
+    ref, n_ref = O.solve_twins(*g, prune=0)
+    tw, n_tw = O.solve_twins(*g, prune=1)
+    assert (tw.count, tw.hash) == (ref.count, ref.hash), name
+    assert n_tw <= n_ref, name
+    if min(g[0], g[1]) <= 20:
+        b = O.brute(*g)
+        assert (tw.count, tw.hash) == (b.count, b.hash), name
+
+
+def test_twin_removal_prunes_repeated_neighbourhoods():
+    g = twin_heavy(120, 80, 6, 0.3, 7)
+    ref, n_ref = O.solve_twins(*g, prune=0)
+    tw, n_tw = O.solve_twins(*g, prune=1)
+    assert (tw.count, tw.hash) == (ref.count, ref.hash)
+    assert n_tw < n_ref / 2, (n_tw, n_ref)
