@@ -9,6 +9,7 @@
 #include <cstring>
 #include <memory>
 #include <mutex>
+#include <new>
 #include <string>
 #include <vector>
 #include <unistd.h>
@@ -670,10 +671,14 @@ mbe_status mbe_solve_list(const mbe_graph* g, mbe_variant variant, const mbe_tun
         if (pos[1] > cap_words)
             return fail(MBE_ERANGE, "output holds " + std::to_string(cap_words) +
                                         " words; the bicliques take " + std::to_string(pos[1]));
-        std::vector<uint64_t> dev(pos[0]);
-        if (pos[0])
-            HIPC(hipMemcpy(dev.data(), em.buf, pos[0] * 8, hipMemcpyDeviceToHost));
-        s = decode_bicliques(p, dev, agg.count, out, pos[1]);
+        try {
+            std::vector<uint64_t> dev(pos[0]);
+            if (pos[0])
+                HIPC(hipMemcpy(dev.data(), em.buf, pos[0] * 8, hipMemcpyDeviceToHost));
+            s = decode_bicliques(p, dev, agg.count, out, pos[1]);
+        } catch (const std::bad_alloc&) {
+            return fail(MBE_ENOMEM, "host memory for the biclique records");
+        }
         if (s != MBE_OK) return s;
     }
     agg.total_ms = now_ms() - t0;
