@@ -33,7 +33,10 @@
 //   forward form (noRS, and small P u Q): the same sets from row tests.
 // Q' drops candidates with no neighbour in L' (they can never cover a
 // descendant's L); a descent whose universe shrank by compact_ratio is
-// re-localised into a nested, compacted problem. Neither removes a tree node.
+// re-localised into a nested, compacted problem; root localisation and every
+// re-localisation keep only Q candidates whose rows are not contained in
+// another kept Q candidate's row (reduce_q), which shrinks the candidate
+// universe several-fold on skewed graphs. None of these removes a tree node.
 #ifndef MBE_EMULATE
 #include <hip/hip_runtime.h>
 #endif
@@ -840,6 +843,120 @@ __device__ void dq_release(const KParams& kp, u64 pos) {
     if (lane() == 0) st_rlx(kp.dq + 4 + (pos % kp.n_slots), pos + kp.n_slots);
 }
 
+// ------------------------------------------------------------ Q reduction
+// Q (candidates already handled) only ever answers one question: does some
+// Q candidate's row contain L'? Every L' of this problem and of its
+// descendants lies in the problem's universe, and a descendant's Q rows are
+// restrictions of these rows, so a Q candidate whose row is contained in
+// another kept Q candidate's row never decides anything. Dropping it leaves
+// every maximality decision, the tree and the bicliques unchanged. At the
+// configuration-1 roots of rank ~400, for example, 8,782 Q candidates hold
+// 1,730 distinct rows, and 339 of them are not contained in another.
+//
+// Rows are taken by popcount, highest first, so that a row can only be
+// contained in rows taken before it; each row is tested against the kept
+// rows through the kept rows' columns (kcols[j] = kept rows holding bit j,
+// kKeepCap of them), a batch of 64 at a time. Rows of one batch are not
+// tested against each other, so a few contained rows may stay: the result
+// is a valid reduction, not always the smallest. Once kKeepCap rows are
+// kept, later survivors are kept without being indexed.
+constexpr uint32_t kKeepCap = 1024;
+constexpr uint32_t kKW = kKeepCap / 64;  // kcols words per universe bit
+constexpr uint32_t kReduceMinQ = 64;     // smaller Q sets are kept whole
+
+// After a wave's own global atomics: make them visible to its later loads.
+__device__ __forceinline__ void sync_mem() {
+    fence_acq();
+    wsync();
+}
+
+// Words of scratch reduce_q needs for nq rows over a d-bit universe.
+__host__ __device__ constexpr size_t reduce_scratch_words(uint32_t d, uint32_t nq) {
+    return (size_t)(d + 2) / 2 + (nq + 1) / 2 + (size_t)d * kKW + 2;
+}
+
+// rows: candidate rows of W words over a d-bit universe, by position; the
+// Q candidates are positions qs[0..nq) (or q0 + i when qs is null). Sets
+// the bit of every kept Q position in keep (cleared by the caller) and
+// returns how many were kept. scratch: reduce_scratch_words(d, nq) words.
+__device__ uint32_t reduce_q(const u64* rows, uint32_t W, uint32_t d, const uint32_t* qs,
+                             uint32_t q0, uint32_t nq, u64* keep, u64* scratch) {
+    uint32_t* hist = (uint32_t*)scratch;                       // d + 1 bins
+    uint32_t* ord = (uint32_t*)(scratch + (d + 2) / 2);        // nq positions
+    u64* kcols = scratch + (d + 2) / 2 + (nq + 1) / 2;         // d * kKW
+    for (uint32_t i = lane(); i <= d; i += 64) hist[i] = 0;
+    for (size_t i = lane(); i < (size_t)d * kKW; i += 64) kcols[i] = 0;
+    sync_mem();
+    // counting sort of the positions by popcount, descending
+    for (uint32_t i = lane(); i < nq; i += 64) {
+        const uint32_t q = qs ? qs[i] : q0 + i;
+        uint32_t pc = 0;
+        for (uint32_t w = 0; w < W; w++) pc += __popcll(rows[(size_t)q * W + w]);
+        atomicAdd(&hist[d - pc], 1u);
+    }
+    sync_mem();
+    uint32_t carry = 0;
+    for (uint32_t b = 0; b <= d; b += 64) {
+        const uint32_t i = b + lane();
+        const uint32_t v = i <= d ? hist[i] : 0;
+        const uint32_t inc = scan_add32(v);
+        if (i <= d) hist[i] = carry + inc - v;
+        carry += rdlane32(inc, 63);
+    }
+    sync_mem();
+    for (uint32_t i = lane(); i < nq; i += 64) {
+        const uint32_t q = qs ? qs[i] : q0 + i;
+        uint32_t pc = 0;
+        for (uint32_t w = 0; w < W; w++) pc += __popcll(rows[(size_t)q * W + w]);
+        ord[atomicAdd(&hist[d - pc], 1u)] = q;
+    }
+    sync_mem();
+    uint32_t nk = 0, kept = 0;
+    for (uint32_t b = 0; b < nq; b += 64) {
+        const uint32_t i = b + lane();
+        const bool valid = i < nq;
+        const uint32_t q = valid ? ord[i] : 0;
+        const u64* r = rows + (size_t)q * W;
+        bool dom = false;
+        if (valid && nk) {
+            const uint32_t nkw = (nk + 63) >> 6;
+            for (uint32_t w = 0; w < nkw && !dom; w++) {
+                u64 a = (w == nkw - 1 && (nk & 63)) ? ((1ull << (nk & 63)) - 1) : ~0ull;
+                for (uint32_t rw = 0; rw < W && a; rw++) {
+                    u64 bits = r[rw];
+                    while (bits && a) {
+                        const uint32_t j = rw * 64 + __builtin_ctzll(bits);
+                        bits &= bits - 1;
+                        a &= kcols[(size_t)j * kKW + w];
+                    }
+                }
+                dom = a != 0;
+            }
+        }
+        const bool surv = valid && !dom;
+        const u64 sm = __ballot(surv);
+        const uint32_t cnt = __popcll(sm);
+        if (surv) atomicOr(&keep[q >> 6], 1ull << (q & 63));
+        if (nk + cnt <= kKeepCap) {
+            if (surv) {
+                const uint32_t t = nk + __popcll(sm & lanemask_lt());
+                for (uint32_t rw = 0; rw < W; rw++) {
+                    u64 bits = r[rw];
+                    while (bits) {
+                        const uint32_t j = rw * 64 + __builtin_ctzll(bits);
+                        bits &= bits - 1;
+                        atomicOr(&kcols[(size_t)j * kKW + (t >> 6)], 1ull << (t & 63));
+                    }
+                }
+            }
+            nk += cnt;
+        }
+        kept += cnt;
+        sync_mem();
+    }
+    return kept;
+}
+
 // Source sets of a compaction: the candidates of P at index >= t stay in P,
 // those below t join Q (they precede the handed-over part), X stays X, and
 // candidate xq (the level's in-progress x, or none) joins Q.
@@ -871,14 +988,45 @@ __device__ __forceinline__ void frame_sizes(const Local& S, const SplitSrc& src,
     n = red_add32(b);
 }
 
-// Compact a source of problem S into the arrays of a smaller universe: L's
-// set bits renumbered 0..dn-1, the candidates renumbered 0..n-1 in order.
-// Writes hl', hc', rows' (dn-bit rows), P', Q', X'. S.list and S.lbits are
+// Scratch of a compaction at the arena's tail, above every live problem:
+// each candidate's row over the new universe, the keep bitmap, the Q
+// positions and reduce_q's scratch.
+struct CompactTmp {
+    u64* trow;     // n * Wn
+    u64* keep;     // (n + 63) / 64
+    uint32_t* qs;  // n
+    u64* rq;       // reduce_scratch_words(dn, n)
+    size_t words;
+};
+__device__ __forceinline__ size_t compact_tmp_words(uint32_t dn, uint32_t n) {
+    return (size_t)n * ((dn + 63) >> 6) + (n + 63) / 64 + (n + 1) / 2 + reduce_scratch_words(dn, n);
+}
+// Place the scratch at the tail of the arena; false if it would reach down
+// to word `floor` (the end of the arena words in use).
+__device__ bool compact_tmp(const KParams& kp, u64* arena, size_t floor, uint32_t dn, uint32_t n,
+                            CompactTmp& t) {
+    t.words = compact_tmp_words(dn, n);
+    if (floor + t.words > kp.arena_words) return false;
+    u64* p = arena + (kp.arena_words - t.words);
+    t.trow = p;
+    p += (size_t)n * ((dn + 63) >> 6);
+    t.keep = p;
+    p += (n + 63) / 64;
+    t.qs = (uint32_t*)p;
+    p += (n + 1) / 2;
+    t.rq = p;
+    return true;
+}
+
+// Compaction of a source of problem S into a smaller universe, phase 1: the
+// source's candidates (S.list, ascending) and L bits (S.lbits), each
+// candidate's row over the L bits renumbered 0..dn-1 (tmp.trow), and the
+// candidates kept (tmp.keep): every P and X candidate and the Q candidates
+// reduce_q keeps. Returns how many are kept. S.list and S.lbits are
 // overwritten (scratch).
-__device__ void compact_sets(const Local& S, const SplitSrc& src, u64* hl2, u64* hc2, u64* rows2,
-                             u64* P2, u64* Q2, u64* X2, uint32_t dn, uint32_t n) {
+__device__ uint32_t compact_prepare(const Local& S, const SplitSrc& src, uint32_t dn, uint32_t n,
+                                    const CompactTmp& tmp) {
     const uint32_t Wn = (dn + 63) >> 6;
-    // candidate list (ascending) and L bit list
     uint32_t n0 = 0;
     for (uint32_t w = 0; w < S.Wc; w++) {
         const u64 cw = uni64(src.cw(w));
@@ -888,20 +1036,15 @@ __device__ void compact_sets(const Local& S, const SplitSrc& src, u64* hl2, u64*
     uint32_t d0 = 0;
     for (uint32_t w = 0; w < S.W; w++) {
         const u64 word = uni64(src.L[w]);
-        if ((word >> lane()) & 1) {
-            const uint32_t at = d0 + __popcll(word & lanemask_lt());
-            S.lbits[at] = w * 64 + lane();
-            hl2[at] = S.hl[w * 64 + lane()];
-        }
+        if ((word >> lane()) & 1) S.lbits[d0 + __popcll(word & lanemask_lt())] = w * 64 + lane();
         d0 += __popcll(word);
     }
     wsync();
-    // candidates: hash terms, rows bit-extracted over L, P'/Q'/X' by ballot
+    uint32_t nq = 0;
     for (uint32_t c0 = 0; c0 < n; c0 += 64) {
         const uint32_t kn = c0 + lane();
         const bool valid = kn < n;
         const uint32_t k = valid ? S.list[kn] : 0;
-        if (valid) hc2[kn] = S.hc[k];
         const u64* r = S.rows + (size_t)k * S.W;
         for (uint32_t w2 = 0; w2 < Wn; w2++) {
             u64 acc = 0;
@@ -910,41 +1053,80 @@ __device__ void compact_sets(const Local& S, const SplitSrc& src, u64* hl2, u64*
                 const uint32_t j = S.lbits[w2 * 64 + t];
                 if (valid) acc |= ((r[j >> 6] >> (j & 63)) & 1ull) << t;
             }
-            if (valid) rows2[(size_t)kn * Wn + w2] = acc;
+            if (valid) tmp.trow[(size_t)kn * Wn + w2] = acc;
         }
-        const uint32_t w = k >> 6, b = k & 63;
-        const bool inP = valid && ((src.pw(w) >> b) & 1);
-        const bool inQ = valid && ((src.qw(w) >> b) & 1);
-        const bool inX = valid && ((src.X[w] >> b) & 1);
-        const u64 bp = __ballot(inP), bq = __ballot(inQ), bx = __ballot(inX);
-        if (lane() == 0) {
-            P2[c0 >> 6] = bp;
-            Q2[c0 >> 6] = bq;
-            X2[c0 >> 6] = bx;
-        }
+        const bool inQ = valid && ((src.qw(k >> 6) >> (k & 63)) & 1);
+        const u64 bq = __ballot(inQ);
+        const u64 bk = __ballot(valid && !inQ);
+        if (lane() == 0) tmp.keep[c0 >> 6] = bk;
+        if (inQ) tmp.qs[nq + __popcll(bq & lanemask_lt())] = kn;
+        nq += __popcll(bq);
     }
-    wsync();
+    sync_mem();
+    if (nq < kReduceMinQ) {
+        for (uint32_t w = lane(); w < (n + 63) / 64; w += 64)
+            tmp.keep[w] = (w == (n - 1) / 64 && (n & 63)) ? ((1ull << (n & 63)) - 1) : ~0ull;
+        sync_mem();
+        return n;
+    }
+    return n - nq + reduce_q(tmp.trow, Wn, dn, tmp.qs, 0, nq, tmp.keep, tmp.rq);
+}
+
+// Phase 2: the kept candidates renumbered 0..nk-1 in order: hl2 (dn terms),
+// hc2, rows2 (Wn words each), and P2, Q2, X2 (Wcn words each, cleared here).
+__device__ void compact_write(const Local& S, const SplitSrc& src, uint32_t dn, uint32_t n,
+                              uint32_t nk, const CompactTmp& tmp, u64* hl2, u64* hc2, u64* rows2,
+                              u64* P2, u64* Q2, u64* X2) {
+    const uint32_t Wn = (dn + 63) >> 6, Wcn = (nk + 63) >> 6;
+    for (uint32_t i = lane(); i < dn; i += 64) hl2[i] = S.hl[S.lbits[i]];
+    for (uint32_t w = lane(); w < Wcn; w += 64) {
+        P2[w] = 0;
+        Q2[w] = 0;
+        X2[w] = 0;
+    }
+    sync_mem();
+    uint32_t at = 0;
+    for (uint32_t c0 = 0; c0 < n; c0 += 64) {
+        const uint32_t kn = c0 + lane();
+        const u64 kw = uni64(tmp.keep[c0 >> 6]);
+        if ((kw >> lane()) & 1) {
+            const uint32_t to = at + __popcll(kw & lanemask_lt());
+            const uint32_t k = S.list[kn];
+            hc2[to] = S.hc[k];
+            for (uint32_t w2 = 0; w2 < Wn; w2++)
+                rows2[(size_t)to * Wn + w2] = tmp.trow[(size_t)kn * Wn + w2];
+            const uint32_t w = k >> 6, b = k & 63;
+            u64* dst = ((src.pw(w) >> b) & 1) ? P2 : ((src.qw(w) >> b) & 1) ? Q2 : X2;
+            atomicOr(&dst[to >> 6], 1ull << (to & 63));
+        }
+        at += __popcll(kw);
+    }
+    sync_mem();
 }
 
 // Write a compaction source into a slot as a LEVEL call over its own
 // universe: [0] graph, [1] d, [2] nc, [3] R hash sum, [4] |R|, then hl[d],
-// hc[nc], rows[nc * W], P[Wc], Q[Wc], X[Wc].
-__device__ void write_snapshot(const Local& S, const SplitSrc& src, u64 sr, u64 nr, uint32_t g,
-                               u64* slot) {
+// hc[nc], rows[nc * W], P[Wc], Q[Wc], X[Wc]. The scratch goes to the tail of
+// the donor's arena (room checked by snapshot_words).
+__device__ void write_snapshot(const KParams& kp, const Local& S, const SplitSrc& src, u64* arena,
+                               size_t floor, u64 sr, u64 nr, uint32_t g, u64* slot) {
     uint32_t dn, n;
     frame_sizes(S, src, dn, n);
-    const uint32_t Wn = (dn + 63) >> 6, Wcn = (n + 63) >> 6;
+    CompactTmp tmp;
+    compact_tmp(kp, arena, floor, dn, n, tmp);
+    const uint32_t nk = compact_prepare(S, src, dn, n, tmp);
+    const uint32_t Wn = (dn + 63) >> 6, Wcn = (nk + 63) >> 6;
     u64* hl2 = slot + 8;
     u64* hc2 = hl2 + dn;
-    u64* rows2 = hc2 + n;
-    u64* P2 = rows2 + (size_t)n * Wn;
+    u64* rows2 = hc2 + nk;
+    u64* P2 = rows2 + (size_t)nk * Wn;
     u64* Q2 = P2 + Wcn;
     u64* X2 = Q2 + Wcn;
-    compact_sets(S, src, hl2, hc2, rows2, P2, Q2, X2, dn, n);
+    compact_write(S, src, dn, n, nk, tmp, hl2, hc2, rows2, P2, Q2, X2);
     if (lane() == 0) {
         slot[0] = g;
         slot[1] = dn;
-        slot[2] = n;
+        slot[2] = nk;
         slot[3] = sr;
         slot[4] = nr;
     }
@@ -1004,41 +1186,48 @@ __device__ bool load_snapshot(const KParams& kp, Local& S, u64* arena, u64* lds,
 __device__ bool compact_child(const KParams& kp, const Local& S, uint32_t k, const u64* Fc,
                               Local& S2, u64* arena, u64* lds, uint32_t dn, uint32_t n,
                               uint32_t pn) {
+    const u64* L = Fc + kFrameHdr;
+    const u64* P = L + S.W;
+    const u64* Q = P + S.Wc;
+    const SplitSrc src{L, P, Q, Q + S.Wc, 0, 0xFFFFFFFFu};
+    CompactTmp tmp;
+    if (!compact_tmp(kp, arena, S.end, dn, n, tmp)) return false;
+    const uint32_t nk = compact_prepare(S, src, dn, n, tmp);
     S2.d = dn;
-    S2.nc = n;
+    S2.nc = nk;
     S2.x = 0xFFFFFFFFu;
     // after S's LDS data and scratch and its live frames (1..k+1)
     const uint32_t lds_from =
         (uint32_t)(S.fbase - lds) + min(k + 1, S.nlds) * S.fs;
-    if (!layout(kp, S2, arena, lds, S.end, lds_from)) return false;
-    const u64* L = Fc + kFrameHdr;
-    const u64* P = L + S.W;
-    const u64* Q = P + S.Wc;
+    if (!layout(kp, S2, arena, lds, S.end, lds_from) || S2.end + tmp.words > kp.arena_words)
+        return false;
     u64* F1 = S2.frame(1);
     u64* L1 = F1 + kFrameHdr;
     u64* P1 = L1 + S2.W;
     u64* Q1 = P1 + S2.Wc;
     u64* X1 = Q1 + S2.Wc;
     const u64 sr = Fc[1], nr = Fc[2];
-    const SplitSrc src{L, P, Q, Q + S.Wc, 0, 0xFFFFFFFFu};
-    compact_sets(S, src, S2.hl, S2.hc, S2.rows, P1, Q1, X1, dn, n);
+    compact_write(S, src, dn, n, nk, tmp, S2.hl, S2.hc, S2.rows, P1, Q1, X1);
     build_cols(S2);
     fill_full_l(S2, L1);
     if (lane() == 0) {
         F1[1] = sr;
         F1[2] = nr;
-        F1[3] = pq(pn, n - pn);
+        F1[3] = pq(pn, nk - pn);
     }
     wsync();
     return true;
 }
 
-// Size in words of a snapshot of the given source, or ~0 when the snapshot
-// or the problem it becomes would not fit a slot or a thief's arena.
-__device__ size_t snapshot_words(const KParams& kp, const Local& S, const SplitSrc& src) {
+// Size in words of a snapshot of the given source (before Q reduction), or
+// ~0 when the snapshot or the problem it becomes would not fit a slot or a
+// thief's arena, or its compaction scratch would not fit above word `floor`
+// of the donor's arena.
+__device__ size_t snapshot_words(const KParams& kp, const Local& S, const SplitSrc& src,
+                                 size_t floor) {
     uint32_t dn, n;
     frame_sizes(S, src, dn, n);
-    if (!dn) return ~(size_t)0;
+    if (!dn || floor + compact_tmp_words(dn, n) > kp.arena_words) return ~(size_t)0;
     const size_t Wn = (dn + 63) >> 6, Wcn = (n + 63) >> 6;
     const size_t words = 8 + (size_t)dn + n + (size_t)n * Wn + 3 * Wcn;
     // the thief's layout with nothing in LDS (see layout())
@@ -1107,7 +1296,7 @@ __device__ bool try_donate(const KParams& kp, const Local& S, uint32_t cur, uint
             const uint32_t keep = pc / 2;  // A: the lowest `keep` candidates
             const uint32_t t = keep ? split_point(T, P, keep) : 0;
             const SplitSrc src{L, P, Q, X, t, running ? (uint32_t)F[0] : 0xFFFFFFFFu};
-            if (snapshot_words(kp, T, src) == ~(size_t)0) continue;
+            if (snapshot_words(kp, T, src, S.end) == ~(size_t)0) continue;
             if (lane() == 0) {  // pending and the graph's open tasks, before publishing
                 atomicAdd(kp.dq + 3, 1ull);
                 if (kp.remaining) atomicAdd(kp.remaining + 2 * (size_t)g, 1ull);
@@ -1121,7 +1310,7 @@ __device__ bool try_donate(const KParams& kp, const Local& S, uint32_t cur, uint
                 return false;
             }
             u64* slot = (u64*)kp.slots + (size_t)((u64)pos % kp.n_slots) * kp.slot_words;
-            write_snapshot(T, src, F[1], F[2], g, slot);
+            write_snapshot(kp, T, src, arena, S.end, F[1], F[2], g, slot);
             dq_publish(kp, (u64)pos);
             for (uint32_t w = lane(); w < T.Wc; w += 64) {
                 const u64 b = src.pw(w);
@@ -1264,7 +1453,14 @@ __device__ void dfs(const KParams& kp, const DevGraph& G, uint32_t g, Local& S, 
 
 // Localise root x (rows/cols over N(x) x distance-two candidates) and
 // process the root node. Returns 0 done, 1 DFS from frame 1 needed, -1 does
-// not fit the arena.
+// not fit the arena (nothing has been reported then, so a later pass with
+// a larger arena can redo the root).
+//
+// Arena: [degree prefix][candidate ranks][the problem (unless it all fits
+// LDS)] ... [kept-Q list][rows of every candidate] at the tail. The rows are
+// built first at the tail; the root node is decided on them; the Q
+// candidates are reduced (reduce_q); then the problem is laid out with the
+// root's P (ranks below x, positions 0..kx-1) followed by the kept Q.
 template <int V>
 __device__ int build_root(const KParams& kp, const DevGraph& G, uint32_t x, u64* arena, u64* lds,
                           u64* mark, uint32_t* cpos, Local& S, u64& cnt, u64& hs, u64& nodes,
@@ -1311,8 +1507,7 @@ __device__ int build_root(const KParams& kp, const DevGraph& G, uint32_t x, u64*
     }
     // this wave's atomics, then its own loads: an acquire-only fence orders
     // them (nothing here is published to other waves)
-    fence_acq();
-    wsync();
+    sync_mem();
     const uint32_t nwords = (G.nR + 63) >> 6;
     uint32_t ncl = 0, kxl = 0;
     for (uint32_t w = lane(); w < nwords; w += 64) {
@@ -1321,16 +1516,20 @@ __device__ int build_root(const KParams& kp, const DevGraph& G, uint32_t x, u64*
         if (w < (x >> 6)) kxl += __popcll(m);
         else if (w == (x >> 6)) kxl += __popcll(m & ((1ull << (x & 63)) - 1));
     }
-    S.nc = red_add32(ncl);
-    S.kx = red_add32(kxl);
-    off = off_cand + (S.nc + 1) / 2;
-    if (!layout(kp, S, arena, lds, off, kLdsHdr)) {
+    const uint32_t nc = red_add32(ncl);
+    const uint32_t kx = red_add32(kxl);
+    S.kx = kx;
+    const uint32_t W = (S.d + 63) >> 6;
+    off = off_cand + (nc + 1) / 2;
+    const size_t trow_words = (size_t)nc * W;
+    const size_t tail_words = trow_words + (nc + 1) / 2 + 1;
+    if (off + max(tail_words, (size_t)S.d) > kp.arena_words) {
         for (uint32_t w = lane(); w < nwords; w += 64) mark[w] = 0;
-        fence_acq();
-        wsync();
+        sync_mem();
         return -1;
     }
-    for (uint32_t j = lane(); j < S.d; j += 64) S.hl[j] = G.hL[S.Nx[j]];
+    u64* trow = arena + (kp.arena_words - trow_words);
+    uint32_t* kidx = (uint32_t*)(arena + (kp.arena_words - tail_words));
 
     // assign local indices in rank order; clear the bitmap
     uint32_t k0 = 0;
@@ -1347,80 +1546,125 @@ __device__ int build_root(const KParams& kp, const DevGraph& G, uint32_t x, u64*
             const uint32_t v = w * 64 + b;
             S.cand[k] = v;
             cpos[v] = k;
-            S.hc[k] = G.hR[v];
             k++;
         }
         k0 += rdlane32(inc, 63);
     }
-    const size_t nz = (size_t)S.nc * S.W + (size_t)S.d * S.Wc;
-    for (size_t i = lane(); i < nz; i += 64) S.rows[i] = 0;
-    fence_acq();
-    wsync();
-    // rows and cols of the local bit matrix
+    for (size_t i = lane(); i < trow_words; i += 64) trow[i] = 0;
+    sync_mem();
+    // the candidates' rows over N(x)
     for (uint32_t f0 = 0; f0 < T; f0 += 64) {
         const uint32_t f = f0 + lane();
         if (f < T) {
             const uint32_t j = find_j(S.pre, S.d, f);
             const uint32_t u = S.Nx[j];
             const uint32_t v = G.adjL[G.offL[u] + f - S.pre[j]];
-            if (v != x) {
-                const uint32_t k = cpos[v];
-                atomicOr(&S.rows[(size_t)k * S.W + (j >> 6)], 1ull << (j & 63));
-                atomicOr(&S.cols[(size_t)j * S.Wc + (k >> 6)], 1ull << (k & 63));
-            }
+            if (v != x) atomicOr(&trow[(size_t)cpos[v] * W + (j >> 6)], 1ull << (j & 63));
         }
     }
-    fence_acq();
-    wsync();
+    sync_mem();
 
-    // ---- root node: L' = N(x), P = cands below kx, Q = the rest
+    // ---- root node: L' = N(x); a full Q row makes it non-maximal, full P
+    // rows are absorbed
+    bool nonmax = false;
+    uint32_t ab = 0;
+    u64 abh = 0;
+    for (uint32_t c0 = 0; c0 < nc && !nonmax; c0 += 64) {
+        const uint32_t k = c0 + lane();
+        bool full = false;
+        if (k < nc) {
+            uint32_t c = 0;
+            for (uint32_t w = 0; w < W; w++) c += __popcll(trow[(size_t)k * W + w]);
+            full = c == S.d;
+        }
+        if (__ballot(full && k >= kx)) nonmax = true;
+        if (full && k < kx) {
+            ab++;
+            abh += G.hR[S.cand[k]];
+        }
+    }
+    if (nonmax) {
+        nodes++;
+        return 0;
+    }
+    ab = red_add32(ab);
+    abh = red_add64(abh);
+    const u64 sr = G.hR[x] + abh, nr = 1 + ab;
+    const bool descend = kx != ab;  // P' nonempty
+    uint32_t nq = nc - kx;
+    if (descend) {
+        // reduce Q (its scratch goes where the problem will be laid out)
+        if (nq >= kReduceMinQ &&
+            off + (nq + 63) / 64 + reduce_scratch_words(S.d, nq) + tail_words <= kp.arena_words) {
+            u64* keep = arena + off;
+            for (uint32_t w = lane(); w < (nq + 63) / 64; w += 64) keep[w] = 0;
+            sync_mem();
+            nq = reduce_q(trow + (size_t)kx * W, W, S.d, nullptr, 0, nq, keep,
+                          keep + (nq + 63) / 64);
+            // kept positions, ascending
+            uint32_t at = 0;
+            for (uint32_t w = 0; w < ((nc - kx) + 63) / 64; w++) {
+                const u64 kw = uni64(keep[w]);
+                if ((kw >> lane()) & 1) kidx[at + __popcll(kw & lanemask_lt())] = kx + w * 64 + lane();
+                at += __popcll(kw);
+            }
+        } else {
+            for (uint32_t i = lane(); i < nq; i += 64) kidx[i] = kx + i;
+        }
+        sync_mem();
+        S.nc = kx + nq;
+        if (!layout(kp, S, arena, lds, off, kLdsHdr) || S.end + tail_words > kp.arena_words) return -1;
+    }
+    // the root biclique (N(x), {x} u absorbed)
+    nodes++;
+    u64 sl = 0;
+    for (uint32_t j = lane(); j < S.d; j += 64) sl += G.hL[S.Nx[j]];
+    sl = red_add64(sl);
+    record(G, sl, sr, S.d, nr, cnt, hs);
+    if (kp.emit_pos) {
+        // the L' terms are the hash terms of N(x) in order
+        u64* hl = arena + off;
+        if (descend) hl = S.hl;
+        for (uint32_t j = lane(); j < S.d; j += 64) hl[j] = G.hL[S.Nx[j]];
+        sync_mem();
+        emit_biclique(kp.emit_pos, kp.emit, kp.emit_cap, hl, nullptr, S.d, nr, sr);
+    }
+    if (!descend) return 0;
+
+    // the problem: hash terms, rows (P then kept Q), columns, frame 1
+    for (uint32_t j = lane(); j < S.d; j += 64) S.hl[j] = G.hL[S.Nx[j]];
+    for (uint32_t k = lane(); k < S.nc; k += 64) {
+        const uint32_t o = k < kx ? k : kidx[k - kx];
+        S.hc[k] = G.hR[S.cand[o]];
+        for (uint32_t w = 0; w < W; w++) S.rows[(size_t)k * W + w] = trow[(size_t)o * W + w];
+    }
+    sync_mem();
+    build_cols(S);
     u64* F1 = S.frame(1);
     u64* L1 = F1 + kFrameHdr;
     u64* P1 = L1 + S.W;
     u64* Q1 = P1 + S.Wc;
     u64* X1 = Q1 + S.Wc;
     fill_full_l(S, L1);
-    for (uint32_t w = lane(); w < S.Wc; w += 64) X1[w] = 0;
-    bool nonmax = false;
-    uint32_t ab = 0;
-    u64 abh = 0;
-    for (uint32_t c0 = 0; c0 < S.nc && !nonmax; c0 += 64) {
+    for (uint32_t c0 = 0; c0 < S.nc; c0 += 64) {
         const uint32_t k = c0 + lane();
-        const bool valid = k < S.nc;
         bool full = false;
-        if (valid) {
+        if (k < kx) {
             uint32_t c = 0;
-            const u64* r = S.rows + (size_t)k * S.W;
-            for (uint32_t w = 0; w < S.W; w++) c += __popcll(r[w]);
+            for (uint32_t w = 0; w < W; w++) c += __popcll(S.rows[(size_t)k * W + w]);
             full = c == S.d;
         }
-        const bool isP = valid && k < S.kx, isQ = valid && k >= S.kx;
-        if (__ballot(full && isQ)) nonmax = true;
-        const u64 bp = __ballot(isP && !full), bq = __ballot(isQ);
+        const u64 bp = __ballot(k < kx && !full), bq = __ballot(k >= kx && k < S.nc);
         if (lane() == 0) {
             P1[c0 >> 6] = bp;
             Q1[c0 >> 6] = bq;
-        }
-        if (full && isP) {
-            ab++;
-            abh += S.hc[k];
+            X1[c0 >> 6] = 0;
         }
     }
-    nodes++;
-    if (nonmax) return 0;
-    ab = red_add32(ab);
-    abh = red_add64(abh);
-    u64 sl = 0;
-    for (uint32_t j = lane(); j < S.d; j += 64) sl += S.hl[j];
-    sl = red_add64(sl);
-    const u64 sr = G.hR[x] + abh, nr = 1 + ab;
-    record(G, sl, sr, S.d, nr, cnt, hs);
-    if (kp.emit_pos) emit_biclique(kp.emit_pos, kp.emit, kp.emit_cap, S.hl, nullptr, S.d, nr, sr);
-    if (S.kx == ab) return 0;  // P' empty
     if (lane() == 0) {
         F1[1] = sr;
         F1[2] = nr;
-        F1[3] = pq(S.kx - ab, S.nc - S.kx);
+        F1[3] = pq(kx - ab, S.nc - kx);
     }
     wsync();
     return 1;

@@ -73,8 +73,10 @@ struct Ring {
 // Largest arena (u64 words) the local problem of any root of p can take:
 // the degree-prefix and candidate lists of root localisation, then hash
 // terms, both bit matrices, scratch lists and a full-depth frame stack (the
-// kernel's layout), with the distance-two candidate count bounded by the sum
-// of the root's neighbours' other edges.
+// kernel's layout, before Q reduction shrinks it), and the rows of every
+// candidate with the kept-Q list that localisation builds at the arena's
+// tail, with the distance-two candidate count bounded by the sum of the
+// root's neighbours' other edges.
 uint64_t root_bound_words(const mbe::Prepared& p) {
     std::vector<uint32_t> degL(p.nL);
     for (uint32_t u = 0; u < p.nL; u++) degL[u] = p.offL[u + 1] - p.offL[u];
@@ -89,7 +91,8 @@ uint64_t root_bound_words(const mbe::Prepared& p) {
         const uint64_t scratch = (nc + 1) / 2 + 1 + (d + 1) / 2 + 1;
         const uint64_t frames = (std::min(d, nc) + 2) * (4 + W + 3 * Wc);
         const uint64_t lists = (d + 2) / 2 + std::max<uint64_t>(d, (nc + 1) / 2) + 1;
-        best = std::max(best, lists + data + scratch + frames);
+        const uint64_t tail = nc * W + (nc + 1) / 2 + 1;
+        best = std::max(best, lists + data + scratch + frames + tail);
     }
     return best;
 }

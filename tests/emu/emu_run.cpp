@@ -1,0 +1,189 @@
+// Runs the engine's persistent enumeration kernel (cumbe_amd/csrc/
+// mbe_kernel.hip, compiled for the CPU with wave_emu.h) on one graph --
+// TEST INFRASTRUCTURE ONLY. Mirrors the host engine's pass loop (buffers,
+// root tasks, overflow passes with larger arenas) in host memory and prints
+//   count <n> hash <0x...> nodes <n> donated <n> stolen <n> launches <n>
+// Usage:
+//   emu_run [options] bi <path>
+//   emu_run [options] cl <n_u> <n_v> <m> <gamma_u> <gamma_v> <max_deg> <seed>
+//   emu_run [options] er <n_u> <n_v> <p> <n_planted> <lo> <hi> <top_frac> <seed>
+//   emu_run [options] batch <seed>
+// Options: -v variant(0..3) -w workers -l lds_words -a arena_words -s slots
+//          -c list_cutoff -r compact_ratio -i donate_interval -t twins(0|2)
+//          -b root_begin -e root_end -k root_stride -S side(-1|0|1)
+#include "wave_emu.h"
+#include "../../cumbe_amd/csrc/mbe_kernel.hip"
+
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../cumbe_amd/csrc/graph.h"
+
+namespace mbe {
+namespace {
+__thread u64 lds_mem[20480];  // 160 KiB: the dynamic LDS of one wave
+}
+}  // namespace mbe
+
+using namespace mbe;
+
+struct Opt {
+    int variant = 0, side = -1;
+    uint32_t workers = 8, lds_words = 3072, slots = 64, cutoff = 0, ratio = 2, interval = 64,
+             twins = 2, rb = 0, re = 0, rs = 1;
+    uint64_t arena_words = 1u << 18;
+};
+
+template <int V>
+static void wave_fn(void* p) { mbe_persistent<V>(*(KParams*)p); }
+
+static void run_pass(KParams& kp, uint32_t workers) {
+    void (*fn)(void*) = kp.variant == 1 ? wave_fn<1> : kp.variant == 2 ? wave_fn<2>
+                      : kp.variant == 3 ? wave_fn<3> : wave_fn<0>;
+    std::vector<std::thread> th;
+    for (uint32_t w = 0; w < workers; w++)
+        th.emplace_back([&kp, w, workers, fn] { emu::run_wave(w, workers, fn, &kp); });
+    for (auto& t : th) t.join();
+}
+
+static void* zalloc(size_t bytes) {
+    void* p = calloc(1, bytes + 256);
+    if (!p) {
+        fprintf(stderr, "emu_run: out of memory (%zu bytes)\n", bytes);
+        exit(3);
+    }
+    return p;
+}
+
+int main(int argc, char** argv) {
+    Opt o;
+    int i = 1;
+    for (; i < argc && argv[i][0] == '-' && argv[i][1] && !isdigit(argv[i][1]); i += 2) {
+        const char f = argv[i][1];
+        const long long v = atoll(argv[i + 1]);
+        switch (f) {
+            case 'v': o.variant = (int)v; break;
+            case 'w': o.workers = (uint32_t)v; break;
+            case 'l': o.lds_words = (uint32_t)v; break;
+            case 'a': o.arena_words = (uint64_t)v; break;
+            case 's': o.slots = (uint32_t)v; break;
+            case 'c': o.cutoff = (uint32_t)v; break;
+            case 'r': o.ratio = (uint32_t)v; break;
+            case 'i': o.interval = (uint32_t)v; break;
+            case 't': o.twins = (uint32_t)v; break;
+            case 'b': o.rb = (uint32_t)v; break;
+            case 'e': o.re = (uint32_t)v; break;
+            case 'k': o.rs = (uint32_t)v; break;
+            case 'S': o.side = (int)v; break;
+            default: fprintf(stderr, "unknown option -%c\n", f); return 2;
+        }
+    }
+    if (i >= argc) return 2;
+    const std::string kind = argv[i];
+    Graph g;
+    std::string err;
+    if (kind == "bi" && i + 1 < argc) {
+        if (!read_bi(argv[i + 1], g, &err)) { fprintf(stderr, "%s\n", err.c_str()); return 2; }
+    } else if (kind == "cl" && i + 7 < argc) {
+        g = gen_chung_lu(atoi(argv[i + 1]), atoi(argv[i + 2]), atoll(argv[i + 3]), atof(argv[i + 4]),
+                         atof(argv[i + 5]), atoi(argv[i + 6]), atoll(argv[i + 7]));
+    } else if (kind == "er" && i + 8 < argc) {
+        g = gen_er_planted(atoi(argv[i + 1]), atoi(argv[i + 2]), atof(argv[i + 3]), atoi(argv[i + 4]),
+                           atoi(argv[i + 5]), atoi(argv[i + 6]), atof(argv[i + 7]), atoll(argv[i + 8]));
+    } else if (kind == "batch" && i + 1 < argc) {
+        g = gen_batch_graph(atoll(argv[i + 1]));
+    } else {
+        fprintf(stderr, "bad graph arguments\n");
+        return 2;
+    }
+    Prepared p;
+    prepare(g, o.side, p);
+    DevGraph dg{};
+    dg.offR = p.offR.data();
+    dg.adjR = p.adjR.data();
+    dg.offL = p.offL.data();
+    dg.adjL = p.adjL.data();
+    dg.hR = p.hR.data();
+    dg.hL = p.hL.data();
+    dg.nR = p.nR;
+    dg.nL = p.nL;
+    dg.n_roots = p.n_roots;
+    dg.r_is_u = p.r_is_u;
+    dg.task_base = 0;
+    std::vector<uint64_t> tasks;
+    const uint64_t end = o.re ? std::min<uint64_t>(o.re, p.n_roots) : p.n_roots;
+    for (uint64_t r = o.rb; r < end; r += o.rs ? o.rs : 1) tasks.push_back(r);
+
+    const uint64_t mark_words = (p.nR + 63) / 64 + 1, cpos_len = (uint64_t)p.nR + 1;
+    const size_t ctl_words = 2 + kStatCount + kSecCount + 2;
+    unsigned long long* ctl = (unsigned long long*)zalloc(ctl_words * 8);
+    unsigned long long* dq = (unsigned long long*)zalloc((4 + o.slots) * 8);
+    unsigned long long abort_flag = 0;
+    uint64_t arena_words = o.arena_words;
+    uint32_t workers = o.workers;
+    unsigned launches = 0;
+    KParams kp{};
+    while (!tasks.empty()) {
+        if (launches > 6) { fprintf(stderr, "root does not fit any arena\n"); return 4; }
+        const uint64_t slot_words = std::min<uint64_t>(std::max<uint64_t>(4096, arena_words), 1u << 19);
+        uint64_t* arena = (uint64_t*)zalloc(arena_words * workers * 8);
+        uint64_t* mark = (uint64_t*)zalloc(mark_words * workers * 8);
+        uint32_t* cpos = (uint32_t*)zalloc(cpos_len * workers * 4);
+        uint64_t* slots = (uint64_t*)zalloc((size_t)o.slots * slot_words * 8);
+        uint64_t* ovf = (uint64_t*)zalloc(tasks.size() * 8);
+        unsigned long long* progress = (unsigned long long*)zalloc(workers * 16);
+        for (uint32_t s = 0; s < o.slots; s++) dq[4 + s] = s;
+        dq[0] = dq[1] = dq[2] = dq[3] = 0;
+        ctl[0] = ctl[1] = 0;  // cursor, overflow count (stats and results accumulate)
+        kp = KParams{};
+        kp.graphs = &dg;
+        kp.n_graphs = 1;
+        kp.variant = (uint32_t)o.variant;
+        kp.n_tasks = tasks.size();
+        kp.task_list = tasks.data();
+        kp.cursor = ctl;
+        kp.overflow_count = ctl + 1;
+        kp.stats = ctl + 2;
+        kp.results = ctl + 2 + kStatCount + kSecCount;
+        kp.arena = arena;
+        kp.arena_words = arena_words;
+        kp.mark = mark;
+        kp.mark_words = mark_words;
+        kp.cpos = cpos;
+        kp.cpos_len = cpos_len;
+        kp.lds_words = o.lds_words;
+        kp.list_cutoff = o.cutoff;
+        kp.compact_ratio = o.ratio;
+        kp.twins = o.twins;
+        kp.abort = &abort_flag;
+        kp.node_count = kp.stats + kStatNodeBudget;
+        kp.overflow_list = ovf;
+        kp.overflow_cap = tasks.size();
+        kp.slots = slots;
+        kp.slot_words = slot_words;
+        kp.n_slots = o.slots;
+        kp.donate_interval = o.interval;
+        kp.dq = dq;
+        kp.progress = progress;
+        run_pass(kp, workers);
+        launches++;
+        const uint64_t novf = ctl[1];
+        std::vector<uint64_t> next(ovf, ovf + novf);
+        free(arena);
+        free(mark);
+        free(cpos);
+        free(slots);
+        free(ovf);
+        free(progress);
+        tasks.swap(next);
+        arena_words *= 16;
+        workers = std::max<uint32_t>(1, std::min<uint32_t>(workers, 4));
+    }
+    const unsigned long long* st = ctl + 2;
+    const unsigned long long* res = ctl + 2 + kStatCount + kSecCount;
+    printf("count %llu hash 0x%016llx nodes %llu donated %llu stolen %llu dropped %llu launches %u\n",
+           res[0], res[1], st[kStatNodes], st[kStatDonated], st[kStatStolen], st[kStatDropped],
+           launches);
+    return st[kStatDropped] ? 5 : 0;
+}
