@@ -4,27 +4,28 @@
 Workload (one "step" = one enumeration pass over it): the one-graph-per-launch
 Chung-Lu graph |U| = 94,000, |V| = 30,000, ~293,000 edges, exponents 2.0 (U)
 and 1.8 (V), degrees capped at 7,600, seed 2 -- restricted to the root tasks
-of rank >= 400 (19,834 of its 20,234 roots). The 400 highest-degree roots span
-the graph's near-complete hub core, whose maximal bicliques number well beyond
-10^9 (their enumeration does not finish in ten minutes on the GPU); the root
-subset is a deterministic, oracle-checkable slice of the same graph and the
-same kernel path.
+of rank >= 400 (19,834 of its 20,234 roots). The highest-degree roots span a
+near-complete hub core: root r alone has exactly 2^r maximal bicliques for
+r <= 16 (211,566 at r = 18), so the whole graph cannot be enumerated; the
+root subset is a deterministic, oracle-checked slice of the same graph and
+the same kernel path.
 
 Reported per JSON line (rank 0, one GPU):
   value       bicliques / kernel seconds (HIP events around the enumeration
-              launches on the engine's stream; the graph is already in HBM)
-  roofline    algorithmic bytes / launch time against 8 TB/s HBM: 8 bytes
-              (one 4-byte neighbour id + one 4-byte position lookup) per
-              adjacency entry W that the canonical serial algorithm scans on
-              this workload (W from the oracle: tests/golden/configs.json);
+              launch on the engine's stream; the graph is already in HBM)
+  roofline    the dominant kernel's algorithmic bytes per launch over the
+              launch time, against 8 TB/s HBM: 8 bytes per 64-bit operand
+              word the tree nodes' set operations read (bit-matrix rows and
+              columns, L/P/Q/X words, counted by the kernel from set sizes)
+              plus 8 bytes per adjacency entry read while localising roots;
               `traffic` is the HBM bytes per launch rocprofv3 measured
-              (profiles/), and `bitmap_GBps` the engine's own operand words
-              (bit-matrix words its nodes read + adjacency entries of root
-              localisation, 8 bytes each) per second
+              (profiles/); `reference_equivalent_GBps` is 8 bytes per
+              adjacency entry the reference's list scans would read (W, from
+              the oracle) per second -- a work rate, not a hardware rate
   cpu_baseline the CPU oracle (the reference algorithm restated serially,
-              root-parallel over host threads) on a stride-64 sample of the
-              roots of rank >= 1000; the GPU solves the same sample and must
-              match its count and hash bit-exactly
+              root-parallel over host threads) on bounded samples of the same
+              roots, all host threads and one core; the GPU solves the same
+              samples and must match their counts and hashes
   batch       the batched workload's graphs whose oracle result is known
               (tests/golden/batch_c3.csv): one captured persistent launch
               (mbe_batch_plan) against one launch per graph (mbe_solve), both
@@ -51,7 +52,11 @@ import cumbe_amd as M  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 WORKLOAD = dict(n_u=94000, n_v=30000, m=293000, gamma_u=2.0, gamma_v=1.8, max_deg=7600, seed=2)
 ROOT_BEGIN = 400
-SAMPLE = dict(begin=1000, end=0, stride=64)
+# CPU-baseline samples of the workload's roots: ranks 528 + 64k on every host
+# thread (about 80 CPU-seconds; ranks 400 and 464 alone would take 180), and
+# ranks 1040 + 64k on one core (about 16 s)
+SAMPLE = dict(begin=528, end=0, stride=64)
+SAMPLE_1CORE = dict(begin=1040, end=0, stride=64)
 # serving's large class (+ seed): 200,000 x 100,000, 1.5M edges, exponent 2.0,
 # degrees capped at 128 (uncapped, its hub core is not enumerable in minutes)
 LARGE = (200000, 100000, 1500000, 2.0, 2.0, 128)
@@ -143,27 +148,60 @@ def traffic_from_profiles():
     return float(m.group(1)) if m else None
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or "unknown"
+
+
 def cpu_baseline(g, gpu_tuning):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O  # the oracle: checker and CPU baseline only
 
     n_u, n_v, off, adj = g.csr()
-    threads = max(1, min(16, os.cpu_count() or 1))
-    r = O.solve_roots(n_u, n_v, off, adj, begin=SAMPLE["begin"], end=SAMPLE["end"],
-                      stride=SAMPLE["stride"], threads=threads)
-    tun = dict(gpu_tuning, root_begin=SAMPLE["begin"], root_end=SAMPLE["end"],
-               root_stride=SAMPLE["stride"])
-    gr = M.solve(g, "cuMBE", tun)
-    parity = (gr.count, gr.hash) == (r.count, r.hash)
+    cpus = sorted(os.sched_getaffinity(0))
+    threads = max(1, min(16, len(cpus)))
+    note(f"cpu baseline: oracle on {threads} threads")
+    ra = O.solve_roots(n_u, n_v, off, adj, begin=SAMPLE["begin"], end=SAMPLE["end"],
+                       stride=SAMPLE["stride"], threads=threads)
+    note("cpu baseline: oracle on one core")
+    os.sched_setaffinity(0, {cpus[0]})
+    try:
+        r1 = O.solve_roots(n_u, n_v, off, adj, begin=SAMPLE_1CORE["begin"],
+                           end=SAMPLE_1CORE["end"], stride=SAMPLE_1CORE["stride"], threads=1)
+    finally:
+        os.sched_setaffinity(0, set(cpus))
+    parity = True
+    gpu = {}
+    for name, smp, ref in (("all_threads", SAMPLE, ra), ("one_core", SAMPLE_1CORE, r1)):
+        tun = dict(gpu_tuning, root_begin=smp["begin"], root_end=smp["end"],
+                   root_stride=smp["stride"])
+        gr = M.solve(g, "cuMBE", tun)
+        parity &= (gr.count, gr.hash) == (ref.count, ref.hash)
+        gpu[name] = gr.count / (gr.kernel_ms / 1000.0)
     return {
-        "value": r.count / max(r.seconds, 1e-9),
+        "value": ra.count / max(ra.seconds, 1e-9),
         "unit": "bicliques/s",
         "cores": threads,
         "kind": "port",
-        "sample": (f"oracle (serial reference algorithm, root-parallel) on the roots of rank "
-                   f">= {SAMPLE['begin']} with stride {SAMPLE['stride']} of the same graph: "
-                   f"{r.count} bicliques in {r.seconds:.2f} s; GPU on the same sample "
-                   f"{'matches' if parity else 'DIFFERS'} (hash {r.hash:#018x})"),
+        "cpu_model": cpu_model(),
+        "sample": (f"oracle (the reference algorithm restated serially, root-parallel on "
+                   f"{threads} threads) on the roots of rank {SAMPLE['begin']} + "
+                   f"{SAMPLE['stride']}k of the same graph: {ra.count} bicliques in "
+                   f"{ra.seconds:.2f} s; GPU on the same roots "
+                   f"{'matches' if parity else 'DIFFERS'} (hash {ra.hash:#018x})"),
+        "one_core": {
+            "value": r1.count / max(r1.seconds, 1e-9),
+            "sample": (f"roots of rank {SAMPLE_1CORE['begin']} + {SAMPLE_1CORE['stride']}k, "
+                       f"one thread pinned to one core: {r1.count} bicliques in "
+                       f"{r1.seconds:.2f} s"),
+        },
+        "gpu_same_samples_bicliques_per_s": gpu,
         "parity": parity,
     }
 
@@ -230,13 +268,19 @@ def main():
 
     value = count / (kernel_ms / 1000.0)
     launch_ms = kernel_ms / max(launches, 1)
-    # algorithmic bytes: 8 per adjacency entry of the canonical algorithm
+    launch_s = launch_ms / 1000.0
+    # algorithmic bytes of the kernel's own formulation per launch (one
+    # launch per step): operand words of the nodes' set operations plus
+    # adjacency entries of root localisation, 8 bytes each
+    op_bytes = 8.0 * (words + build) / max(launches, 1)
+    achieved = op_bytes / launch_s / 1e9
+    # the reference's list scans on the same roots: 8 bytes per adjacency
+    # entry W the canonical serial algorithm scans (W from the oracle)
     wl = golden("configs.json")["c1"]["bench_workload"]
     w_alg = wl["work"] if args.root_begin == wl["begin"] else None
-    # W is per launch (one launch per step): divide by one launch's time
-    achieved = (8.0 * w_alg / (launch_ms / 1000.0) / 1e9) if w_alg else None
-    bitmap_gbps = 8.0 * (words + build) / (kernel_ms / 1000.0) / 1e9
+    ref_gbps = 8.0 * w_alg / launch_s / 1e9 if w_alg else None
     traffic = traffic_from_profiles()
+    twins = not (tun.get("flags", 0) & M.FLAG_REFERENCE_TREE)
     line = {
         "metric": "maximal bicliques/s (bit-exact)",
         "value": value,
@@ -259,7 +303,11 @@ def main():
         "config": {
             "workload": (f"Chung-Lu {n_u}x{n_v}, {n_e} edges, gamma 2.0/1.8, max degree 7600, "
                          f"seed 2; root tasks of rank >= {args.root_begin}"),
+            "deviation": ("the 400 highest-degree roots are left out: they span a near-complete "
+                          "hub core whose root r has 2^r maximal bicliques (exact for r <= 16)"),
             "variant": args.variant,
+            "twin_pruning": twins,
+            "q_reduction": True,
             "tuning": tun,
         },
         "count": first[0],
@@ -270,12 +318,15 @@ def main():
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS if achieved else None,
+            "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "work_entries": w_alg,
-            "work_source": wl["work_source"],
-            "bitmap_GBps": bitmap_gbps,
+            "traffic_GBps": traffic / launch_s / 1e9 if traffic else None,
+            "algorithmic_bytes_per_launch": op_bytes,
+            "limiter": ("latency of dependent LDS and L2 accesses (three single-wave workers "
+                        "per CU, LDS-limited); the operand words are served mostly from LDS"),
             "launch_ms": launch_ms,
+            "reference_equivalent_GBps": ref_gbps,
+            "reference_work_entries": w_alg,
         },
         "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(g, base),
     }

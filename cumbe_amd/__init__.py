@@ -26,7 +26,7 @@ VARIANTS = {"cuMBE": 0, "noRS": 1, "noES": 2, "noWS": 3}
 STATUS = {
     0: "ok", -1: "invalid argument", -2: "out of memory", -3: "HIP error",
     -4: "timeout", -5: "over admission budget", -6: "I/O error", -7: "busy",
-    -8: "not ready",
+    -8: "not ready", -9: "output buffer too small", -10: "internal self-check failed",
 }
 
 

@@ -107,7 +107,9 @@ def test_engine_path_variants_and_rotation():
 
 def footprint(g):
     """Admission footprint as the server defines it: packed CSR bytes plus
-    the largest local problem (u64 words) any root can build."""
+    the largest local problem (u64 words) any root can build, with the rows
+    of every candidate and the kept-Q list that localisation builds at the
+    tail of the arena."""
     import numpy as np
     n_u, n_v, off, adj = g.csr()
     du = np.diff(off).astype(np.int64)
@@ -124,7 +126,8 @@ def footprint(g):
     W, Wc = (d + 63) // 64, (nc + 63) // 64
     words = (d + nc + nc * W + d * Wc + (nc + 1) // 2 + 1 + (d + 1) // 2 + 1 +
              (np.minimum(d, nc) + 2) * (4 + W + 3 * Wc) +
-             (d + 2) // 2 + np.maximum(d, (nc + 1) // 2) + 1)
+             (d + 2) // 2 + np.maximum(d, (nc + 1) // 2) + 1 +
+             nc * W + (nc + 1) // 2 + 1)
     slot = lambda b: (b + 8 + 255) & ~255
     E = int(off[-1])
     packed = (slot((nR + 1) * 4) + slot(E * 4) + slot((nL + 1) * 4) + slot(E * 4) +
