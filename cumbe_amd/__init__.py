@@ -77,6 +77,12 @@ class Result(C.Structure):
         return d
 
 
+class PackedGraph(C.Structure):
+    """mbe_packed_graph: layout of a graph packed by mbe_graph_pack."""
+    _fields_ = [("n_r", C.c_uint32), ("n_l", C.c_uint32), ("n_roots", C.c_uint32),
+                ("r_is_u", C.c_uint32), ("off", C.c_uint64 * 6), ("bytes", C.c_uint64)]
+
+
 SECTION_NAMES = ["other", "root_build", "select", "lprime", "check", "expand", "report",
                  "compact", "donate", "fetch", "wait"]
 FLAG_SECTIONS = 1
@@ -115,6 +121,9 @@ def lib() -> C.CDLL:
     L.mbe_solve.argtypes = [G, C.c_int, P(Tuning), P(Result)]
     L.mbe_solve_list.argtypes = [G, C.c_int, P(Tuning), P(C.c_uint32), C.c_uint64,
                                  P(C.c_uint64), P(Result)]
+    L.mbe_init.argtypes = []
+    L.mbe_graph_pack.argtypes = [G, C.c_int, C.c_void_p, C.c_uint64, P(PackedGraph)]
+    L.mbe_solve_noalloc.argtypes = [C.c_void_p, P(PackedGraph), C.c_int, P(Tuning), P(Result)]
     L.mbe_batch_solve.argtypes = [P(G), C.c_size_t, C.c_int, P(Tuning), P(Result), P(Result)]
     L.mbe_batch_plan_create.argtypes = [P(G), C.c_size_t, C.c_int, P(Tuning), P(G)]
     L.mbe_batch_plan_run.argtypes = [G, P(Result), P(Result)]
@@ -131,6 +140,7 @@ def lib() -> C.CDLL:
     for name in ("mbe_graph_load_bi", "mbe_graph_write_bi", "mbe_graph_from_edge_list",
                  "mbe_graph_from_csr", "mbe_graph_info", "mbe_graph_csr", "mbe_gen_chung_lu",
                  "mbe_gen_er_planted", "mbe_gen_batch_graph", "mbe_solve", "mbe_solve_list",
+                 "mbe_init", "mbe_graph_pack", "mbe_solve_noalloc",
                  "mbe_batch_solve",
                  "mbe_batch_plan_create", "mbe_batch_plan_run", "mbe_server_create",
                  "mbe_server_submit", "mbe_server_poll", "mbe_server_get_stats"):
@@ -215,6 +225,15 @@ class Graph:
         _check(lib().mbe_graph_info(self._h, C.byref(nu), C.byref(nv), C.byref(ne)))
         return nu.value, nv.value, ne.value
 
+    def pack(self, side: int = -1):
+        """(bytes as a numpy uint8 array, PackedGraph): the graph prepared and
+        packed for mbe_solve_noalloc (mbe_graph_pack)."""
+        info = PackedGraph()
+        _check(lib().mbe_graph_pack(self._h, side, None, 0, C.byref(info)))
+        buf = np.zeros(max(info.bytes, 1), dtype=np.uint8)
+        _check(lib().mbe_graph_pack(self._h, side, buf.ctypes.data, info.bytes, C.byref(info)))
+        return buf[:info.bytes], info
+
     def csr(self):
         """(n_u, n_v, off, adj) as numpy copies."""
         nu, nv, ne = self.info()
@@ -249,6 +268,21 @@ def solve(graph: Graph, variant: str = "cuMBE", tuning: Optional[dict] = None,
     r.partial = st == -4
     if not (allow_partial and r.partial):
         _check(st)
+    return r
+
+
+def init() -> None:
+    """mbe_init: set the engine up on device 0 (needed before solve_noalloc)."""
+    _check(lib().mbe_init())
+
+
+def solve_noalloc(dev_graph: int, info: PackedGraph, variant: str = "cuMBE",
+                  tuning: Optional[dict] = None) -> Result:
+    """mbe_solve_noalloc on a packed graph at device address dev_graph."""
+    r = Result()
+    t = _tuning(tuning)
+    _check(lib().mbe_solve_noalloc(C.c_void_p(dev_graph), C.byref(info), VARIANTS[variant],
+                                   C.byref(t) if t else None, C.byref(r)))
     return r
 
 

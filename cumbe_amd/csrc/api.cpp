@@ -53,6 +53,9 @@ struct Engine {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     Bufs b;
     unsigned long long* abort_flag = nullptr;  // host-mapped watchdog flag
+    // device address of the allocation-free solve's static scratch array
+    // (taken once, with the engine's device current)
+    unsigned long long* scratch = nullptr;
 
     mbe_status ensure_init() {
         if (init) return MBE_OK;
@@ -68,6 +71,7 @@ struct Engine {
         HIPC(hipEventCreate(&ev0));
         HIPC(hipEventCreate(&ev1));
         HIPC(hipHostMalloc((void**)&abort_flag, 64, hipHostMallocMapped | hipHostMallocCoherent));
+        HIPC(mbe::noalloc_scratch_address((void**)&scratch));
         init = true;
         return MBE_OK;
     }
@@ -683,6 +687,206 @@ mbe_status mbe_solve_list(const mbe_graph* g, mbe_variant variant, const mbe_tun
     }
     agg.total_ms = now_ms() - t0;
     if (result) *result = agg;
+    return MBE_OK;
+}
+
+mbe_status mbe_init(void) {
+    Engine& E = engine();
+    std::lock_guard<std::mutex> lock(E.mu);
+    return E.ensure_init();
+}
+
+mbe_status mbe_graph_pack(const mbe_graph* g, int side, void* out, uint64_t cap_bytes,
+                          mbe_packed_graph* info) {
+    if (!g || !info || side < -1 || side > 1 || (!out && cap_bytes))
+        return fail(MBE_EINVAL, "bad argument");
+    mbe::Prepared p;
+    mbe::prepare(g->g, side, p);
+    const size_t sizes[6] = {p.offR.size() * 4, p.adjR.size() * 4, p.offL.size() * 4,
+                             p.adjL.size() * 4, p.hR.size() * 8, p.hL.size() * 8};
+    const void* src[6] = {p.offR.data(), p.adjR.data(), p.offL.data(),
+                          p.adjL.data(), p.hR.data(), p.hL.data()};
+    mbe_packed_graph inf{};
+    inf.n_r = p.nR;
+    inf.n_l = p.nL;
+    inf.n_roots = p.n_roots;
+    inf.r_is_u = p.r_is_u ? 1 : 0;
+    uint64_t o = 0;
+    for (int i = 0; i < 6; i++) {
+        inf.off[i] = o;
+        o += upload_slot(sizes[i]);
+    }
+    inf.bytes = o;
+    *info = inf;
+    if (!out) return MBE_OK;
+    if (cap_bytes < inf.bytes)
+        return fail(MBE_ERANGE, "the packed graph takes " + std::to_string(inf.bytes) + " bytes");
+    memset(out, 0, inf.bytes);
+    for (int i = 0; i < 6; i++)
+        if (sizes[i]) memcpy((char*)out + inf.off[i], src[i], sizes[i]);
+    return MBE_OK;
+}
+
+mbe_status mbe_solve_noalloc(const void* dev_graph, const mbe_packed_graph* info,
+                             mbe_variant variant, const mbe_tuning* tuning, mbe_result* out) {
+    if (!dev_graph || !info || !out || (unsigned)variant > 3u) return fail(MBE_EINVAL, "bad argument");
+    for (int i = 0; i < 6; i++)
+        if (info->off[i] % 256 || info->off[i] >= info->bytes)
+            return fail(MBE_EINVAL, "packed graph offsets out of range");
+    if (info->n_roots > info->n_r) return fail(MBE_EINVAL, "packed graph has more roots than vertices");
+    const double t0 = now_ms();
+    Engine& E = engine();
+    std::lock_guard<std::mutex> lock(E.mu);
+    if (!E.init) return fail(MBE_EINVAL, "mbe_solve_noalloc needs mbe_init() at set-up");
+    Tune tu = resolve(tuning, E.cus, E.lds_per_cu);
+    if (tu.flags & MBE_FLAG_VARIANCE)
+        return fail(MBE_EINVAL, "per-worker busy times need a host buffer; not without allocation");
+    if (!(tuning && tuning->arena_words)) tu.arena_words = kNoAllocArenaWords;
+    const bool fixed_workers = tuning && tuning->workers;
+    mbe_status s = ensure_lds(tu.lds_words);
+    if (s != MBE_OK) return s;
+
+    // root range of this call: ranks root_begin, +stride, ... below root_end
+    const uint64_t rend = tu.root_end ? std::min<uint64_t>(tu.root_end, info->n_roots) : info->n_roots;
+    uint64_t n_tasks = tu.root_begin < rend ? (rend - tu.root_begin + tu.root_stride - 1) / tu.root_stride : 0;
+    mbe::DevGraph dg{};
+    const char* base = (const char*)dev_graph;
+    dg.offR = (const uint32_t*)(base + info->off[0]);
+    dg.adjR = (const uint32_t*)(base + info->off[1]);
+    dg.offL = (const uint32_t*)(base + info->off[2]);
+    dg.adjL = (const uint32_t*)(base + info->off[3]);
+    dg.hR = (const uint64_t*)(base + info->off[4]);
+    dg.hL = (const uint64_t*)(base + info->off[5]);
+    dg.nR = info->n_r;
+    dg.nL = info->n_l;
+    dg.n_roots = info->n_roots;
+    dg.r_is_u = info->r_is_u;
+    dg.task_base = 0;
+
+    // scratch words: control block, descriptor, deque, two task lists (the
+    // overflow list of one pass is the task list of the next), slots, then
+    // per worker an arena, a mark bitmap and a rank table
+    Meta m;
+    m.ng = 1;
+    m.max_nr = std::max<uint32_t>(1, info->n_r);
+    m.total_tasks = n_tasks;
+    auto up4 = [](uint64_t w) { return (w + 31) & ~(uint64_t)31; };  // 256-byte steps
+    const uint64_t w_ctl = up4(m.ctl_words()), w_graph = up4(sizeof(mbe::DevGraph) / 8 + 1),
+                   w_dq = up4(4 + kNoAllocSlots), w_list = up4(std::max<uint64_t>(n_tasks, 1));
+    unsigned long long* ctl = E.scratch;
+    mbe::DevGraph* gdev = (mbe::DevGraph*)(E.scratch + w_ctl);
+    unsigned long long* dq = E.scratch + w_ctl + w_graph;
+    uint64_t* lists = (uint64_t*)(dq + w_dq);
+    const uint64_t fixed = w_ctl + w_graph + w_dq + 2 * w_list;
+    if (n_tasks == 0) {
+        *out = mbe_result{};
+        out->total_ms = now_ms() - t0;
+        return MBE_OK;
+    }
+    __atomic_store_n(E.abort_flag, 0ull, __ATOMIC_SEQ_CST);
+    const double deadline = tu.time_limit_ms > 0 ? now_ms() + tu.time_limit_ms : 0;
+    uint64_t arena_words = tu.arena_words;
+    uint32_t workers = tu.workers;
+    float kernel_ms = 0;
+    uint32_t launches = 0, used_workers = 0;
+    mbe::KParams kp{};
+    for (int pass = 0; n_tasks > 0; pass++) {
+        if (pass > 6) return fail(MBE_ERANGE, "root subproblem does not fit an arena of the scratch");
+        const uint64_t slot_words = std::min<uint64_t>(std::max<uint64_t>(4096, arena_words), 1u << 19);
+        const uint64_t w_slots = up4(kNoAllocSlots * slot_words);
+        const uint64_t per_worker = up4(arena_words) + up4(m.mark_words()) + up4((m.cpos_len() + 1) / 2);
+        const uint64_t fit = fixed + w_slots < mbe::kNoAllocWords
+                                 ? (mbe::kNoAllocWords - fixed - w_slots) / per_worker : 0;
+        if (fixed_workers && pass == 0 && workers > fit)
+            return fail(MBE_ERANGE, std::to_string(workers) + " workers need " +
+                                        std::to_string((fixed + w_slots + workers * per_worker) * 8) +
+                                        " bytes of scratch; the allocation-free solve has " +
+                                        std::to_string(mbe::kNoAllocBytes));
+        const uint32_t nw = (uint32_t)std::min<uint64_t>(workers, fit);
+        if (nw == 0)
+            return fail(MBE_ERANGE, "the graph's buffers exceed the allocation-free solve's " +
+                                        std::to_string(mbe::kNoAllocBytes) + " bytes of scratch");
+        uint64_t* slots = lists + 2 * w_list;
+        uint64_t* arena = slots + w_slots;
+        uint64_t* mark = arena + (uint64_t)nw * up4(arena_words);
+        uint32_t* cpos = (uint32_t*)(mark + (uint64_t)nw * up4(m.mark_words()));
+        kp = mbe::KParams{};
+        kp.graphs = gdev;
+        kp.n_graphs = 1;
+        kp.variant = (uint32_t)variant;
+        kp.n_tasks = n_tasks;
+        kp.task_list = pass == 0 ? nullptr : lists + ((pass - 1) & 1) * w_list;
+        kp.task_first = tu.root_begin;
+        kp.task_step = tu.root_stride;
+        kp.cursor = ctl;
+        kp.overflow_count = ctl + 1;
+        kp.stats = ctl + 2;
+        kp.sections = (tu.flags & MBE_FLAG_SECTIONS) ? ctl + 2 + mbe::kStatCount : nullptr;
+        kp.results = ctl + 2 + mbe::kStatCount + mbe::kSecCount;
+        kp.arena = arena;
+        kp.arena_words = up4(arena_words);
+        kp.mark = mark;
+        kp.mark_words = up4(m.mark_words());
+        kp.cpos = cpos;
+        kp.cpos_len = 2 * up4((m.cpos_len() + 1) / 2);
+        kp.lds_words = tu.lds_words;
+        kp.list_cutoff = tu.list_cutoff;
+        kp.compact_ratio = tu.compact_ratio;
+        kp.twins = (tu.flags & MBE_FLAG_REFERENCE_TREE) ? 0 : 2;
+        HIPC(hipHostGetDevicePointer((void**)&kp.abort, E.abort_flag, 0));
+        kp.node_count = kp.stats + mbe::kStatNodeBudget;
+        kp.node_budget = tu.max_nodes;
+        kp.overflow_list = lists + (pass & 1) * w_list;
+        kp.overflow_cap = n_tasks;
+        kp.donate_depth = tu.donate_depth;
+        kp.donate_interval = tu.donate_interval;
+        kp.slots = slots;
+        kp.slot_words = slot_words;
+        kp.n_slots = kNoAllocSlots;
+        kp.dq = dq;
+        HIPC(hipEventRecord(E.ev0, E.stream));
+        if (pass == 0) HIPC(mbe::launch_put_graph(gdev, dg, E.stream));
+        HIPC(enqueue_pass(kp, nw, pass == 0 ? m.ctl_words() : 2, E.stream));
+        HIPC(hipEventRecord(E.ev1, E.stream));
+        for (;;) {
+            const hipError_t q = hipEventQuery(E.ev1);
+            if (q == hipSuccess) break;
+            if (q != hipErrorNotReady) HIPC(q);
+            usleep(100);
+            if (deadline > 0 && now_ms() > deadline) __atomic_store_n(E.abort_flag, 1ull, __ATOMIC_SEQ_CST);
+        }
+        float ms = 0;
+        HIPC(hipEventElapsedTime(&ms, E.ev0, E.ev1));
+        kernel_ms += ms;
+        launches++;
+        used_workers = std::max(used_workers, nw);
+        unsigned long long novf = 0;
+        HIPC(hipMemcpy(&novf, kp.overflow_count, 8, hipMemcpyDeviceToHost));
+        if (__atomic_load_n(E.abort_flag, __ATOMIC_SEQ_CST) || !novf) break;
+        n_tasks = novf;
+        arena_words *= 16;
+        workers = nw;
+    }
+    unsigned long long st[mbe::kStatCount], sc[mbe::kSecCount], rr[2];
+    HIPC(hipMemcpy(st, kp.stats, sizeof(st), hipMemcpyDeviceToHost));
+    HIPC(hipMemcpy(sc, kp.stats + mbe::kStatCount, sizeof(sc), hipMemcpyDeviceToHost));
+    HIPC(hipMemcpy(rr, kp.results, sizeof(rr), hipMemcpyDeviceToHost));
+    mbe_result r{};
+    r.count = rr[0];
+    r.hash = rr[1];
+    r.kernel_ms = kernel_ms;
+    r.nodes = st[mbe::kStatNodes];
+    r.donated = st[mbe::kStatDonated];
+    r.launches = launches;
+    r.workers = used_workers;
+    for (int i = 0; i < mbe::kSecCount; i++) r.sections[i] = sc[i];
+    r.words = st[mbe::kStatWords];
+    r.build_words = st[mbe::kStatBuildWords];
+    r.total_ms = now_ms() - t0;
+    *out = r;
+    if (__atomic_load_n(E.abort_flag, __ATOMIC_SEQ_CST))
+        return fail(MBE_ETIMEOUT, "watchdog: node budget or time limit reached; results partial");
+    if (st[mbe::kStatDropped]) return fail(MBE_ENOMEM, "donated subtree did not fit a worker arena");
     return MBE_OK;
 }
 

@@ -51,6 +51,12 @@ enum KSec : int {
     kSecCount = 11
 };
 
+// Device scratch of the allocation-free solve (mbe_solve_noalloc): one
+// static array per device, partitioned per call into control words, the
+// deque, task lists and per-worker arenas, marks and rank tables.
+constexpr uint64_t kNoAllocBytes = 256ull << 20;
+constexpr uint64_t kNoAllocWords = kNoAllocBytes / 8;
+
 // Fixed per-worker LDS header (ES minima, nest stack, section counters) in
 // u64 words; a worker's LDS budget is at least this plus one frame.
 constexpr uint32_t kLdsHeaderWords = 344;
@@ -88,6 +94,8 @@ struct KParams {
     uint32_t variant;           // mbe_variant
     uint64_t n_tasks;           // tasks in this launch
     const uint64_t* task_list;  // nullable: explicit global task ids
+    uint64_t task_first;        // without a list: task t is task_first + t * task_step
+    uint64_t task_step;         // (0 reads as 1)
     unsigned long long* cursor; // next task index (atomic)
     unsigned long long* results;// 2 per graph: count, hash
     unsigned long long* stats;  // kStatCount

@@ -21,6 +21,8 @@ hipError_t launch_mbe(const KParams& kp, uint32_t workers, hipStream_t stream);
 hipError_t launch_dq_init(unsigned long long* dq, uint32_t n_slots, hipStream_t stream);
 hipError_t launch_fill(unsigned long long* p, uint64_t n, unsigned long long v, hipStream_t stream);
 hipError_t set_lds_limit(uint32_t lds_words);
+hipError_t noalloc_scratch_address(void** ptr);
+hipError_t launch_put_graph(DevGraph* dst, const DevGraph& g, hipStream_t stream);
 
 namespace host {
 
@@ -105,6 +107,9 @@ struct Meta {
 };
 
 constexpr uint32_t kSlots = 1024;  // deque slots
+// allocation-free solve: deque slots and default arena (u64 words)
+constexpr uint32_t kNoAllocSlots = 64;
+constexpr uint64_t kNoAllocArenaWords = 1u << 15;
 
 Tune resolve(const mbe_tuning* t, int cus, size_t lds_per_cu);
 

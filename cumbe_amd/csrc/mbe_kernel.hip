@@ -1886,7 +1886,8 @@ __global__ __launch_bounds__(64) void mbe_persistent(KParams kp) {
                 if (idle && lane() == 0) atomicAdd(kp.dq + 2, ~0ull);
                 idle = false;
                 backoff = 1;
-                const uint64_t task = kp.task_list ? kp.task_list[t] : t;
+                const uint64_t task = kp.task_list ? kp.task_list[t]
+                                                   : kp.task_first + t * (kp.task_step ? kp.task_step : 1);
                 const uint32_t g = find_graph(kp, kp.serve ? serve_graphs(kp) : kp.n_graphs, task);
                 const DevGraph G = kp.graphs[g];
                 const uint32_t x = (uint32_t)(task - G.task_base);
@@ -2017,6 +2018,11 @@ __global__ void mbe_fill(unsigned long long* p, uint64_t n, unsigned long long v
         p[i] = v;
 }
 
+// Store one graph descriptor (a kernel argument) into device memory.
+__global__ void mbe_put_graph(DevGraph* dst, DevGraph g) {
+    if (threadIdx.x == 0) *dst = g;
+}
+
 // Reset the deque control block: head, tail, idle, pending, slot sequences.
 __global__ void mbe_dq_init(unsigned long long* dq, uint32_t n_slots) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2027,6 +2033,20 @@ __global__ void mbe_dq_init(unsigned long long* dq, uint32_t n_slots) {
 }  // namespace
 
 #ifndef MBE_EMULATE
+// The allocation-free solve's scratch (mbe_solve_noalloc). Host code never
+// uses this name as an address: it takes the device address once per device
+// from noalloc_scratch_address() and passes that to the kernels.
+static __device__ __align__(256) unsigned long long scratch[kNoAllocWords];
+
+hipError_t noalloc_scratch_address(void** ptr) {
+    return hipGetSymbolAddress(ptr, HIP_SYMBOL(scratch));
+}
+
+hipError_t launch_put_graph(DevGraph* dst, const DevGraph& g, hipStream_t stream) {
+    hipLaunchKernelGGL(mbe_put_graph, dim3(1), dim3(64), 0, stream, dst, g);
+    return hipGetLastError();
+}
+
 // Host-side launcher: one persistent launch of `workers` single-wave
 // workgroups on `stream`.
 hipError_t launch_mbe(const KParams& kp, uint32_t workers, hipStream_t stream) {

@@ -171,6 +171,55 @@ mbe_status mbe_solve_list(const mbe_graph* g, mbe_variant variant, const mbe_tun
                           uint32_t* out, uint64_t cap_words, uint64_t* used_words,
                           mbe_result* result);
 
+/* Initialise the engine on device 0 (stream, events, the watchdog flag and
+ * the address of the allocation-free solve's scratch). Every entry point
+ * but mbe_solve_noalloc does this on its first call; a caller that cannot
+ * allocate later calls it once at set-up. */
+mbe_status mbe_init(void);
+
+/* A prepared graph packed into one buffer: the enumerated side R relabelled
+ * by root order, both CSRs with sorted lists and the per-vertex hash terms,
+ * each array at a 256-byte aligned byte offset. */
+typedef struct {
+    uint32_t n_r, n_l;       /* enumerated side, other side */
+    uint32_t n_roots;        /* ranks [0, n_roots) have a nonzero degree */
+    uint32_t r_is_u;         /* 1 when R holds the file's rows */
+    uint64_t off[6];         /* byte offsets of offR, adjR, offL, adjL, hR, hL */
+    uint64_t bytes;          /* bytes the packed graph takes */
+} mbe_packed_graph;
+
+/* Prepare g (side: -1 the reference's choice, 0 enumerate U, 1 V) and write
+ * it packed into out. With out NULL and cap_bytes 0 only *info is filled;
+ * MBE_ERANGE if cap_bytes < info->bytes. The caller copies the buffer to
+ * device memory it owns for mbe_solve_noalloc. */
+mbe_status mbe_graph_pack(const mbe_graph* g, int side, void* out, uint64_t cap_bytes,
+                          mbe_packed_graph* info);
+
+/* Bytes of the allocation-free solve's scratch, one static array per device. */
+#define MBE_NOALLOC_SCRATCH_BYTES (256ull << 20)
+
+/* mbe_solve without any allocation. No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).
+ *
+ * dev_graph is device memory holding what mbe_graph_pack wrote (described
+ * by *info); the caller owns it and the result. Every other buffer the
+ * kernels use -- control words, the deque and its slots, task lists, the
+ * per-worker arenas, marks and rank tables -- is carved out of one static
+ * device array of MBE_NOALLOC_SCRATCH_BYTES, so the call makes no hipMalloc,
+ * hipFree or other allocating call. mbe_init must have run (MBE_EINVAL
+ * otherwise). Defaults: 32 K-word (256 KiB) arenas and as many workers as
+ * fit, up to mbe_solve's count; roots whose local problem exceeds an arena
+ * run in follow-up passes with 16 times larger arenas and fewer workers
+ * inside the same array. A tuning or graph whose buffers do not fit the
+ * array returns MBE_ERANGE; MBE_FLAG_VARIANCE returns MBE_EINVAL. The array
+ * is shared by every call on a device and keeps the last call's contents
+ * (each call clears what must start at zero); calls are serialised by the
+ * engine's lock within a process, and calls that overlap on one device
+ * from several processes race. */
+mbe_status mbe_solve_noalloc(const void* dev_graph, const mbe_packed_graph* info,
+                             mbe_variant variant, const mbe_tuning* tuning, mbe_result* out);
+
 /* Many graphs in one persistent launch sharing one task pool; per_graph[i]
  * receives graph i's count and hash (timing fields are the batch's).
  * No reference counterpart: the reference runs one graph per process. */
