@@ -53,6 +53,8 @@ class Tuning(C.Structure):
         ("root_stride", C.c_uint32),
         ("max_nodes", C.c_uint64),
         ("time_limit_ms", C.c_double),
+        ("sparse_ratio", C.c_uint32),
+        ("reserved0", C.c_uint32),
     ]
 
 
@@ -83,8 +85,10 @@ class PackedGraph(C.Structure):
                 ("r_is_u", C.c_uint32), ("off", C.c_uint64 * 6), ("bytes", C.c_uint64)]
 
 
-SECTION_NAMES = ["other", "root_build", "select", "lprime", "check", "expand", "report",
-                 "compact", "donate", "fetch", "wait"]
+# mbe_result.sections, in the slots of the reference's DESECTION clocks
+# (src/header.cuh:25-32, grouped by utils/plot_section.py:7-8)
+SECTION_NAMES = ["other", "compact", "donate", "lprime", "check", "expand", "report",
+                 "select", "root_build", "fetch", "wait"]
 FLAG_SECTIONS = 1
 FLAG_VARIANCE = 2
 FLAG_REFERENCE_TREE = 4   # walk exactly the reference tree (no twin pruning)

@@ -104,6 +104,7 @@ Tune resolve(const mbe_tuning* t, int cus, size_t lds_per_cu) {
     r.max_nodes = t ? t->max_nodes : 0;
     r.time_limit_ms = t ? t->time_limit_ms : 0;
     r.donate_interval = (t && t->donate_interval) ? t->donate_interval : 64;
+    r.sparse_ratio = (t && t->sparse_ratio) ? (t->sparse_ratio == UINT32_MAX ? 0 : t->sparse_ratio) : 8;
     return r;
 }
 
@@ -228,7 +229,8 @@ mbe_status setup_pass(Bufs& B, hipStream_t stream, unsigned long long* abort_hos
     kp.lds_words = tu.lds_words;
     kp.list_cutoff = tu.list_cutoff;
     kp.compact_ratio = tu.compact_ratio;
-    kp.twins = (tu.flags & MBE_FLAG_REFERENCE_TREE) ? 0 : 2;
+    kp.sparse_ratio = tu.sparse_ratio;
+    kp.twins = (tu.flags & MBE_FLAG_REFERENCE_TREE) ? 0 : 1;
     HIPC(hipHostGetDevicePointer((void**)&kp.abort, abort_host, 0));
     kp.node_count = kp.stats + mbe::kStatNodeBudget;
     kp.node_budget = tu.max_nodes;
@@ -832,7 +834,8 @@ mbe_status mbe_solve_noalloc(const void* dev_graph, const mbe_packed_graph* info
         kp.lds_words = tu.lds_words;
         kp.list_cutoff = tu.list_cutoff;
         kp.compact_ratio = tu.compact_ratio;
-        kp.twins = (tu.flags & MBE_FLAG_REFERENCE_TREE) ? 0 : 2;
+        kp.sparse_ratio = tu.sparse_ratio;
+        kp.twins = (tu.flags & MBE_FLAG_REFERENCE_TREE) ? 0 : 1;
         HIPC(hipHostGetDevicePointer((void**)&kp.abort, E.abort_flag, 0));
         kp.node_count = kp.stats + mbe::kStatNodeBudget;
         kp.node_budget = tu.max_nodes;

@@ -35,18 +35,22 @@ enum KStat : int {
     kStatCount = 12
 };
 
-// Kernel time sections (DESECTION equivalent).
+// Kernel time sections, in the slots of the reference's DESECTION clocks
+// (src/header.cuh:25-32) as its plot script groups them
+// (utils/plot_section.py:7-8): 3 L' construction, 4 maximality checking,
+// 5 maximality expansion, 7 candidate selection, 8 subtree fetching, 2 + 9
+// work stealing, the rest "others".
 enum KSec : int {
     kSecOther = 0,    // loop control, frame bookkeeping
-    kSecBuild = 1,    // root localisation (distance-two scan, bit matrices)
-    kSecSelect = 2,   // candidate selection (ES)
+    kSecCompact = 1,  // child re-localisation
+    kSecDonate = 2,   // work stealing: donation (snapshot + push)
     kSecLPrime = 3,   // L' construction
     kSecCheck = 4,    // maximality check of non-maximal nodes
     kSecExpand = 5,   // maximality check + expansion of maximal nodes
     kSecReport = 6,   // biclique hash and count
-    kSecCompact = 7,  // child re-localisation
-    kSecDonate = 8,   // work donation (snapshot + push)
-    kSecFetch = 9,    // stolen subtree load
+    kSecSelect = 7,   // candidate selection (ES)
+    kSecBuild = 8,    // subtree fetching: root claim and localisation
+    kSecFetch = 9,    // work stealing: stolen subtree load
     kSecWait = 10,    // idle: deque empty, waiting for work or termination
     kSecCount = 11
 };
@@ -107,9 +111,10 @@ struct KParams {
     uint32_t* cpos;             // workers * cpos_len (R rank -> local index)
     uint64_t cpos_len;
     uint32_t lds_words;         // dynamic LDS per worker (u64 words)
-    uint32_t list_cutoff;       // |P|+|Q| at or below which a node tests rows (list form)
+    uint32_t list_cutoff;       // |P|+|Q| at or below which a node tests rows (forward form)
+    uint32_t sparse_ratio;      // list-mode roots: bitmap words > ratio * local adjacency (0 = never)
     uint32_t compact_ratio;     // re-localise a child whose universe shrank this much (0 = never)
-    uint32_t twins;             // 1: drop twin candidates of the selected vertex from P
+    uint32_t twins;             // nonzero: drop twin candidates of the selected vertex from P
     // roots that did not fit the arena
     unsigned long long* overflow_count;
     uint64_t* overflow_list;

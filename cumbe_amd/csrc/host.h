@@ -93,6 +93,7 @@ struct Tune {
     uint32_t root_begin, root_end, root_stride;
     uint64_t max_nodes;
     double time_limit_ms;
+    uint32_t sparse_ratio;  // 0 = never list mode
 };
 
 // What a launch needs to know about the uploaded graphs.

@@ -130,15 +130,18 @@ int main(int argc, char** argv) {
     mbe_graph_info(g, &nu, &nv, &ne);
     uint32_t nr = nu <= nv ? nu : nv, nl = nu <= nv ? nv : nu;  // src/main.cu:65-70
 
-    // MBE_SECTIONS=1 adds the per-section time split, like the reference's
-    // DESECTION build (src/header.cuh:25-32, src/main.cu:208-216)
+    // MBE_SECTIONS=1, or running as mbe_desection, adds the per-section time
+    // split, like the reference's DESECTION build (src/header.cuh:25-32,
+    // src/main.cu:208-216; its harness runs ./bin/mbe_desection,
+    // utils/run.py:45-57)
+    const std::string self = std::string(argv[0]).substr(std::string(argv[0]).rfind('/') + 1);
     const char* sec_env = getenv("MBE_SECTIONS");
-    const bool sections = sec_env && atoi(sec_env) != 0;
-    // MBE_VARIANCE=1 prints each worker's busy ticks between the header and
-    // the result lines, like the reference's DEVARIANCE build
-    // (src/mbe_noES.cuh:49-55, read by utils/plot_variance.py)
+    const bool sections = (sec_env && atoi(sec_env) != 0) || self == "mbe_desection";
+    // MBE_VARIANCE=1, or running as mbe_devariance, prints each worker's busy
+    // ticks between the header and the result lines, like the reference's
+    // DEVARIANCE build (src/mbe_noES.cuh:49-55, read by utils/plot_variance.py)
     const char* var_env = getenv("MBE_VARIANCE");
-    const bool variance = var_env && atoi(var_env) != 0;
+    const bool variance = (var_env && atoi(var_env) != 0) || self == "mbe_devariance";
     mbe_tuning tun{};
     tun.side = -1;
     tun.flags = (sections ? MBE_FLAG_SECTIONS : 0) | (variance ? MBE_FLAG_VARIANCE : 0);

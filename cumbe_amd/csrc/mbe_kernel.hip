@@ -194,6 +194,14 @@ struct Local {
     uint32_t x, d, W, nc, Wc, kx, fs, nlds, maxdepth;
     uint32_t lds_off, lds_end;   // LDS words [lds_off, lds_end) in use
     bool in_lds;                 // whole problem in LDS
+    // list mode (a sparse root problem, layout_sparse): rows holds only the
+    // root's P candidates (positions below kx); the candidates of each L
+    // bit and the L bits of each candidate are lists instead of bit matrices
+    bool sparse;
+    uint32_t* lofs;              // d + 1: L bit j's candidates are lcand[lofs[j], lofs[j+1])
+    uint32_t* lcand;
+    uint32_t* rofs;              // nc + 1: candidate k's L bits are rl[rofs[k], rofs[k+1])
+    uint32_t* rl;
     size_t base, end;            // arena words [base, end) in use
     const uint32_t* Nx;
     uint32_t* pre;
@@ -222,6 +230,7 @@ __device__ __forceinline__ u64 pq(uint32_t p, uint32_t q) { return (u64)p | ((u6
 // frames as fit in LDS.
 __device__ bool layout(const KParams& kp, Local& S, u64* arena, u64* lds, size_t off,
                        uint32_t lds_off) {
+    S.sparse = false;
     S.W = (S.d + 63) >> 6;
     S.Wc = (S.nc + 63) >> 6;
     S.maxdepth = min(S.d, S.nc) + 2;
@@ -260,6 +269,60 @@ __device__ bool layout(const KParams& kp, Local& S, u64* arena, u64* lds, size_t
     lp += (size_t)S.nlds * S.fs;
     S.hstack = ap;
     ap += (size_t)(S.maxdepth - S.nlds) * S.fs;
+    S.lds_end = (uint32_t)(lp - lds);
+    S.end = (size_t)(ap - arena);
+    return S.end <= kp.arena_words;
+}
+
+// Place a list-mode root problem (d, nc, kx set; ntl = entries of each of
+// the two lists, or ~0 to read it back from lofs[d]) from arena word off:
+// hash terms, the P candidates' rows, both list offsets, then the scratch
+// lists and as many frames as fit in LDS, the other frames, then the lists.
+// P and X only ever hold the root's P candidates, so a search is at most
+// min(d, kx) + 2 levels deep.
+__device__ bool layout_sparse(const KParams& kp, Local& S, u64* arena, u64* lds, size_t off,
+                              uint32_t lds_off, uint32_t ntl) {
+    S.sparse = true;
+    S.in_lds = false;
+    S.W = (S.d + 63) >> 6;
+    S.Wc = (S.nc + 63) >> 6;
+    S.maxdepth = min(S.d, S.kx) + 2;
+    S.fs = kFrameHdr + S.W + 3 * S.Wc;
+    S.lds_off = lds_off;
+    S.base = off;
+    S.cols = nullptr;
+    u64* ap = arena + off;
+    S.hl = ap;
+    ap += S.d;
+    S.hc = ap;
+    ap += S.nc;
+    S.rows = ap;
+    ap += (size_t)S.kx * S.W;
+    S.lofs = (uint32_t*)ap;
+    ap += (S.d + 2) / 2;
+    S.rofs = (uint32_t*)ap;
+    ap += (S.nc + 2) / 2;
+    if (ntl == 0xFFFFFFFFu) ntl = S.lofs[S.d];
+    const size_t room = lds_off < kp.lds_words ? kp.lds_words - lds_off : 0;
+    const size_t scratch = (S.nc + 1) / 2 + 1 + (S.d + 1) / 2 + 1;
+    u64* lp = lds + lds_off;
+    const bool scratch_lds = scratch <= room;
+    u64* p = scratch_lds ? lp : ap;
+    S.list = (uint32_t*)p;
+    p += (S.nc + 1) / 2 + 1;
+    S.lbits = (uint32_t*)p;
+    p += (S.d + 1) / 2 + 1;
+    if (scratch_lds) lp = p; else ap = p;
+    const size_t lroom = room - (size_t)(lp - (lds + lds_off));
+    S.nlds = (uint32_t)min((size_t)S.maxdepth, lroom / S.fs);
+    S.fbase = lp;
+    lp += (size_t)S.nlds * S.fs;
+    S.hstack = ap;
+    ap += (size_t)(S.maxdepth - S.nlds) * S.fs;
+    S.lcand = (uint32_t*)ap;
+    ap += (ntl + 1) / 2;
+    S.rl = (uint32_t*)ap;
+    ap += (ntl + 1) / 2;
     S.lds_end = (uint32_t)(lp - lds);
     S.end = (size_t)(ap - arena);
     return S.end <= kp.arena_words;
@@ -674,6 +737,123 @@ __device__ NodeOut process_fwd(const Local& S, u64* F, u64* Fc, uint32_t nl) {
     return o;
 }
 
+// List-mode maximality check and expansion (a sparse root problem): the
+// reference's list scans (src/mbe_cuMBE.cuh:216-316) narrowed by two facts.
+// A Q candidate containing L' is adjacent to the L' bit j* with the fewest
+// candidates, so only j*'s candidates are tested, each by a wave-wide scan
+// of its L-bit list against L'. Absorption and P' concern the root's P
+// candidates only, whose rows are bitmaps. Q' is Q (the check of a
+// descendant only looks at candidates adjacent to its own L'). With prune
+// set, twins of the selected vertex (rows within L equal to L', see
+// prune_twins) leave P in the same pass. `scanned` adds the list entries
+// and row words read.
+__device__ __noinline__ NodeOut process_sparse(const Local& S, u64* F, u64* Fc, uint32_t nl,
+                                               bool prune, u64& scanned) {
+    const u64* L = F + kFrameHdr;
+    u64* P = F + kFrameHdr + S.W;
+    const u64* Q = P + S.Wc;
+    const u64* X = Q + S.Wc;
+    const u64* Lc = Fc + kFrameHdr;
+    u64* Pc = Fc + kFrameHdr + S.W;
+    u64* Qc = Pc + S.Wc;
+    u64* Xc = Qc + S.Wc;
+    NodeOut o{true, false, 0, 0, 0, 0, 0, 0, false};
+    uint32_t bl = 0xFFFFFFFFu, bj = 0;
+    for (uint32_t l = lane(); l < nl; l += 64) {
+        const uint32_t j = S.lbits[l];
+        const uint32_t len = S.lofs[j + 1] - S.lofs[j];
+        if (len < bl) {
+            bl = len;
+            bj = j;
+        }
+    }
+    const uint32_t mn = red_min32(bl);
+    const uint32_t js = rdlane32(bj, __builtin_ctzll(__ballot(bl == mn)));
+    const uint32_t c0 = S.lofs[js], c1 = S.lofs[js + 1];
+    u64 sc = c1 - c0;
+    bool nonmax = false;
+    for (uint32_t t = c0; t < c1 && !nonmax; t++) {
+        const uint32_t q = S.lcand[t];
+        if (!((Q[q >> 6] >> (q & 63)) & 1)) continue;
+        const uint32_t r0 = S.rofs[q], r1 = S.rofs[q + 1];
+        if (r1 - r0 < nl) continue;
+        sc += r1 - r0;
+        uint32_t hits = 0;
+        for (uint32_t e0 = r0; e0 < r1; e0 += 64) {
+            const uint32_t e = e0 + lane();
+            bool hit = false;
+            if (e < r1) {
+                const uint32_t j = S.rl[e];
+                hit = (Lc[j >> 6] >> (j & 63)) & 1;
+            }
+            hits += __popcll(__ballot(hit));
+        }
+        nonmax = hits == nl;
+    }
+    o.maximal = !nonmax;
+    if (nonmax && !prune) {
+        scanned += sc;
+        wsync();
+        return o;
+    }
+    // the root's P candidates: absorption and P' (maximal nodes), twins
+    const uint32_t Wp = (S.kx + 63) >> 6;
+    uint32_t ab = 0, pnc = 0, removed = 0;
+    u64 abh = 0;
+    for (uint32_t w = 0; w < Wp; w++) {
+        const u64 pw = uni64(P[w]), all = pw | uni64(X[w]);
+        u64 pn = 0, absd = 0, twins = 0;
+        if (all) {
+            const bool valid = (all >> lane()) & 1;
+            uint32_t c = 0, cl = 0;
+            if (valid) {
+                const u64* r = S.rows + (size_t)(w * 64 + lane()) * S.W;
+                for (uint32_t i = 0; i < S.W; i++) {
+                    const u64 rw = r[i];
+                    c += __popcll(rw & Lc[i]);
+                    cl += __popcll(rw & L[i]);
+                }
+            }
+            absd = __ballot(valid && c == nl);
+            pn = __ballot(valid && c != 0 && c != nl);
+            twins = prune ? __ballot(valid && c == nl && cl == nl && ((pw >> lane()) & 1)) : 0;
+            if ((absd >> lane()) & 1) abh += S.hc[w * 64 + lane()];
+            sc += (u64)__popcll(all) * S.W;
+        }
+        if (lane() == 0) {
+            if (!nonmax) Pc[w] = pn;
+            if (twins) P[w] = pw & ~twins;
+        }
+        ab += __popcll(absd);
+        pnc += __popcll(pn);
+        removed += __popcll(twins);
+    }
+    if (nonmax) {
+        if (lane() == 0 && removed) F[3] -= removed;
+        scanned += sc;
+        wsync();
+        return o;
+    }
+    uint32_t qn = 0;
+    for (uint32_t w = lane(); w < S.Wc; w += 64) {
+        const u64 q = Q[w];
+        Qc[w] = q;
+        Xc[w] = 0;
+        if (w >= Wp) Pc[w] = 0;
+        qn += __popcll(q);
+    }
+    if (lane() == 0 && removed) F[3] -= removed;  // |P| is the low half
+    o.ab = ab;
+    o.abh = red_add64(abh);
+    o.pn = pnc;
+    o.qn = red_add32(qn);
+    o.un = pnc + o.qn;
+    o.descend = pnc != 0;
+    scanned += sc + Wp * S.W;
+    wsync();
+    return o;
+}
+
 // Twin pruning (the candidate removal of iMBEA, Zhang et al., BMC
 // Bioinformatics 2014, which the reference leaves out). A candidate v left in
 // P with N(v) n L == L' = N(x) n L would, selected later at this level, give
@@ -1040,13 +1220,33 @@ __device__ uint32_t compact_prepare(const Local& S, const SplitSrc& src, uint32_
         d0 += __popcll(word);
     }
     wsync();
+    if (S.sparse) {
+        // list mode: each candidate's row from the candidate lists of the new
+        // universe's L bits (S.list is ascending: binary search)
+        for (size_t i = lane(); i < (size_t)n * Wn; i += 64) tmp.trow[i] = 0;
+        sync_mem();
+        for (uint32_t t = lane(); t < dn; t += 64) {
+            const uint32_t j = S.lbits[t];
+            for (uint32_t e = S.lofs[j]; e < S.lofs[j + 1]; e++) {
+                const uint32_t k = S.lcand[e];
+                uint32_t lo = 0, hi = n;
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (S.list[mid] < k) lo = mid + 1; else hi = mid;
+                }
+                if (lo < n && S.list[lo] == k)
+                    atomicOr(&tmp.trow[(size_t)lo * Wn + (t >> 6)], 1ull << (t & 63));
+            }
+        }
+        sync_mem();
+    }
     uint32_t nq = 0;
     for (uint32_t c0 = 0; c0 < n; c0 += 64) {
         const uint32_t kn = c0 + lane();
         const bool valid = kn < n;
         const uint32_t k = valid ? S.list[kn] : 0;
         const u64* r = S.rows + (size_t)k * S.W;
-        for (uint32_t w2 = 0; w2 < Wn; w2++) {
+        for (uint32_t w2 = 0; w2 < Wn && !S.sparse; w2++) {
             u64 acc = 0;
             const uint32_t lim = min(64u, dn - w2 * 64);
             for (uint32_t t = 0; t < lim; t++) {
@@ -1142,6 +1342,7 @@ __device__ bool load_snapshot(const KParams& kp, Local& S, u64* arena, u64* lds,
     S.d = (uint32_t)slot[1];
     S.nc = (uint32_t)slot[2];
     S.x = 0xFFFFFFFFu;
+    S.kx = 0;  // kx is a root problem's field; nest records keep it
     if (!layout(kp, S, arena, lds, kArenaHdr, kLdsHdr)) return false;
     const u64* hl2 = slot + 8;
     const u64* hc2 = hl2 + S.d;
@@ -1196,6 +1397,7 @@ __device__ bool compact_child(const KParams& kp, const Local& S, uint32_t k, con
     S2.d = dn;
     S2.nc = nk;
     S2.x = 0xFFFFFFFFu;
+    S2.kx = 0;
     // after S's LDS data and scratch and its live frames (1..k+1)
     const uint32_t lds_from =
         (uint32_t)(S.fbase - lds) + min(k + 1, S.nlds) * S.fs;
@@ -1243,7 +1445,11 @@ __device__ __forceinline__ void unpack(const KParams& kp, const u64* rec, Local&
     T.d = (uint32_t)rec[1];
     T.nc = (uint32_t)(rec[1] >> 32);
     T.x = 0xFFFFFFFFu;
-    layout(kp, T, arena, lds, (size_t)rec[0], (uint32_t)rec[2]);
+    T.kx = (uint32_t)(rec[3] >> 32) & 0x7FFFFFFFu;
+    if (rec[3] >> 63)
+        layout_sparse(kp, T, arena, lds, (size_t)rec[0], (uint32_t)rec[2], 0xFFFFFFFFu);
+    else
+        layout(kp, T, arena, lds, (size_t)rec[0], (uint32_t)rec[2]);
     k = (uint32_t)(rec[2] >> 32);
     depth0 = (uint32_t)rec[3];
 }
@@ -1390,13 +1596,19 @@ __device__ void dfs(const KParams& kp, const DevGraph& G, uint32_t g, Local& S, 
         const uint32_t nl = make_lprime(S, L, xk, Fc + kFrameHdr);
         sec.mark(kSecLPrime);
         const bool fwd = V == kNoRS || pc - 1 + qc <= kp.list_cutoff;
-        const NodeOut o = fwd ? process_fwd(S, F, Fc, nl) : process_rs(S, F, Fc, nl);
-        // operand words of this node: selection rows and P words, x's row
-        // and L, the L' columns (or the P and Q rows) with P/Q/X, L' terms
-        words += (u64)pc * S.W + S.Wc + 2 * S.W + nl +
-                 (fwd ? (u64)(pc - 1 + qc) * S.W + 3 * S.Wc : (u64)S.Wc * (nl + 3));
-        // twins: 1 = only where the reverse scan already gave T, 2 = everywhere
-        if (pc > 1 && (kp.twins > 1 || (kp.twins == 1 && o.has_tw))) {
+        NodeOut o;
+        if (S.sparse) {  // list-mode root problem: the same check in every variant
+            u64 sc = 0;
+            o = process_sparse(S, F, Fc, nl, pc > 1 && kp.twins, sc);
+            words += (u64)pc * S.W + S.Wc + 2 * S.W + nl + sc + 3 * S.Wc;
+        } else {
+            o = fwd ? process_fwd(S, F, Fc, nl) : process_rs(S, F, Fc, nl);
+            // operand words of this node: selection rows and P words, x's row
+            // and L, the L' columns (or the P and Q rows) with P/Q/X, L' terms
+            words += (u64)pc * S.W + S.Wc + 2 * S.W + nl +
+                     (fwd ? (u64)(pc - 1 + qc) * S.W + 3 * S.Wc : (u64)S.Wc * (nl + 3));
+        }
+        if (pc > 1 && kp.twins && !S.sparse) {
             const u64 r = prune_twins(F, Fc + kFrameHdr, S.cols, S.rows, S.lbits, S.list, S.W,
                                       S.Wc, nl, o.tw, o.has_tw);
             words += (o.has_tw ? 0 : (u64)S.Wc * (nl + 1)) + (r >> 32) * S.W;
@@ -1428,7 +1640,8 @@ __device__ void dfs(const KParams& kp, const DevGraph& G, uint32_t g, Local& S, 
                             r[0] = S.base;
                             r[1] = (u64)S.d | ((u64)S.nc << 32);
                             r[2] = (u64)S.lds_off | ((u64)k << 32);
-                            r[3] = depth0;
+                            r[3] = depth0 | ((u64)(S.kx & 0x7FFFFFFFu) << 32) |
+                                   ((u64)S.sparse << 63);
                         }
                         wsync();
                         np++;
@@ -1449,6 +1662,113 @@ __device__ void dfs(const KParams& kp, const DevGraph& G, uint32_t g, Local& S, 
         }
         wsync();
     }
+}
+
+// List-mode localisation of root x (the candidates are numbered, S.cand and
+// cpos hold them): the candidates of each L bit (N(u_j) without x, in rank
+// order, from the degree prefix), the L bits of each candidate (a counting
+// transpose), the bit rows of the root's P candidates, then the root node.
+// Returns like build_root.
+__device__ __noinline__ int build_root_sparse(const KParams& kp, const DevGraph& G, uint32_t x,
+                                              u64* arena, u64* lds, const uint32_t* cpos,
+                                              Local& S, uint32_t T, uint32_t nc, uint32_t kx,
+                                              size_t off, u64& cnt, u64& hs, u64& nodes) {
+    S.nc = nc;
+    S.kx = kx;
+    const uint32_t ntl = T - S.d;  // every N(u_j) holds x once
+    if (!layout_sparse(kp, S, arena, lds, off, kLdsHdr, ntl)) return -1;
+    const uint32_t W = S.W;
+    for (uint32_t j = lane(); j < S.d; j += 64) S.hl[j] = G.hL[S.Nx[j]];
+    for (uint32_t k = lane(); k < nc; k += 64) S.hc[k] = G.hR[S.cand[k]];
+    for (uint32_t j = lane(); j <= S.d; j += 64) S.lofs[j] = S.pre[j] - j;
+    for (size_t i = lane(); i < (size_t)kx * W; i += 64) S.rows[i] = 0;
+    for (uint32_t k = lane(); k <= nc; k += 64) S.list[k] = 0;  // per-candidate counts
+    sync_mem();
+    for (uint32_t f0 = 0; f0 < T; f0 += 64) {
+        const uint32_t f = f0 + lane();
+        if (f < T) {
+            const uint32_t j = find_j(S.pre, S.d, f);
+            const uint32_t u = S.Nx[j];
+            const uint32_t t = f - S.pre[j];
+            const uint32_t v = G.adjL[G.offL[u] + t];
+            if (v != x) {
+                const uint32_t k = cpos[v];
+                S.lcand[S.lofs[j] + (v < x ? t : t - 1)] = k;
+                atomicAdd(&S.list[k], 1u);
+                if (k < kx) atomicOr(&S.rows[(size_t)k * W + (j >> 6)], 1ull << (j & 63));
+            }
+        }
+    }
+    sync_mem();
+    uint32_t carry = 0;
+    for (uint32_t c0 = 0; c0 <= nc; c0 += 64) {
+        const uint32_t k = c0 + lane();
+        const uint32_t v = k < nc ? S.list[k] : 0;
+        const uint32_t inc = scan_add32(v);
+        if (k <= nc) {
+            S.rofs[k] = carry + inc - v;
+            S.list[k] = carry + inc - v;  // scatter cursor
+        }
+        carry += rdlane32(inc, 63);
+    }
+    sync_mem();
+    for (uint32_t f0 = 0; f0 < T; f0 += 64) {
+        const uint32_t f = f0 + lane();
+        if (f < T) {
+            const uint32_t j = find_j(S.pre, S.d, f);
+            const uint32_t u = S.Nx[j];
+            const uint32_t v = G.adjL[G.offL[u] + f - S.pre[j]];
+            if (v != x) S.rl[atomicAdd(&S.list[cpos[v]], 1u)] = j;
+        }
+    }
+    sync_mem();
+    // root node: L' = N(x); a candidate adjacent to all of it is full
+    bool nonmax = false;
+    uint32_t ab = 0;
+    u64 abh = 0;
+    for (uint32_t c0 = 0; c0 < nc && !nonmax; c0 += 64) {
+        const uint32_t k = c0 + lane();
+        const bool full = k < nc && S.rofs[k + 1] - S.rofs[k] == S.d;
+        if (__ballot(full && k >= kx)) nonmax = true;
+        if (full && k < kx) {
+            ab++;
+            abh += S.hc[k];
+        }
+    }
+    nodes++;
+    if (nonmax) return 0;
+    ab = red_add32(ab);
+    abh = red_add64(abh);
+    const u64 sr = G.hR[x] + abh, nr = 1 + ab;
+    u64 sl = 0;
+    for (uint32_t j = lane(); j < S.d; j += 64) sl += S.hl[j];
+    sl = red_add64(sl);
+    record(G, sl, sr, S.d, nr, cnt, hs);
+    if (kp.emit_pos) emit_biclique(kp.emit_pos, kp.emit, kp.emit_cap, S.hl, nullptr, S.d, nr, sr);
+    if (kx == ab) return 0;  // P' empty
+    u64* F1 = S.frame(1);
+    u64* L1 = F1 + kFrameHdr;
+    u64* P1 = L1 + S.W;
+    u64* Q1 = P1 + S.Wc;
+    u64* X1 = Q1 + S.Wc;
+    fill_full_l(S, L1);
+    for (uint32_t c0 = 0; c0 < nc; c0 += 64) {
+        const uint32_t k = c0 + lane();
+        const bool full = k < kx && S.rofs[k + 1] - S.rofs[k] == S.d;
+        const u64 bp = __ballot(k < kx && !full), bq = __ballot(k >= kx && k < nc);
+        if (lane() == 0) {
+            P1[c0 >> 6] = bp;
+            Q1[c0 >> 6] = bq;
+            X1[c0 >> 6] = 0;
+        }
+    }
+    if (lane() == 0) {
+        F1[1] = sr;
+        F1[2] = nr;
+        F1[3] = pq(kx - ab, nc - kx);
+    }
+    wsync();
+    return 1;
 }
 
 // Localise root x (rows/cols over N(x) x distance-two candidates) and
@@ -1521,15 +1841,11 @@ __device__ int build_root(const KParams& kp, const DevGraph& G, uint32_t x, u64*
     S.kx = kx;
     const uint32_t W = (S.d + 63) >> 6;
     off = off_cand + (nc + 1) / 2;
-    const size_t trow_words = (size_t)nc * W;
-    const size_t tail_words = trow_words + (nc + 1) / 2 + 1;
-    if (off + max(tail_words, (size_t)S.d) > kp.arena_words) {
+    if (off > kp.arena_words) {
         for (uint32_t w = lane(); w < nwords; w += 64) mark[w] = 0;
         sync_mem();
         return -1;
     }
-    u64* trow = arena + (kp.arena_words - trow_words);
-    uint32_t* kidx = (uint32_t*)(arena + (kp.arena_words - tail_words));
 
     // assign local indices in rank order; clear the bitmap
     uint32_t k0 = 0;
@@ -1550,6 +1866,17 @@ __device__ int build_root(const KParams& kp, const DevGraph& G, uint32_t x, u64*
         }
         k0 += rdlane32(inc, 63);
     }
+    // list mode when the bit matrices would dwarf the local adjacency
+    if (kp.sparse_ratio &&
+        (size_t)nc * W + (size_t)S.d * ((nc + 63) >> 6) > (size_t)kp.sparse_ratio * T) {
+        sync_mem();
+        return build_root_sparse(kp, G, x, arena, lds, cpos, S, T, nc, kx, off, cnt, hs, nodes);
+    }
+    const size_t trow_words = (size_t)nc * W;
+    const size_t tail_words = trow_words + (nc + 1) / 2 + 1;
+    if (off + max(tail_words, (size_t)S.d) > kp.arena_words) return -1;
+    u64* trow = arena + (kp.arena_words - trow_words);
+    uint32_t* kidx = (uint32_t*)(arena + (kp.arena_words - tail_words));
     for (size_t i = lane(); i < trow_words; i += 64) trow[i] = 0;
     sync_mem();
     // the candidates' rows over N(x)

@@ -79,6 +79,15 @@ typedef struct {
      * result holds the partial count, hash and node total. */
     uint64_t max_nodes;
     double time_limit_ms;
+    /* List-versus-bitmap cutoff of root localisation: a root whose bit
+     * matrices would take more than sparse_ratio 64-bit words per adjacency
+     * entry of its local graph is kept as sorted adjacency lists instead
+     * (the reference's list scans, src/mbe_cuMBE.cuh:188-236,345-390): bit
+     * rows only for its P candidates, the candidates of each L vertex and
+     * the L vertices of each candidate as lists. 0 = default (8),
+     * UINT32_MAX = never. */
+    uint32_t sparse_ratio;
+    uint32_t reserved0;
 } mbe_tuning;
 
 typedef struct {
@@ -91,11 +100,13 @@ typedef struct {
     uint32_t launches;     /* kernel launches used */
     uint32_t workers;      /* workers used */
     /* Time per kernel section summed over workers, in 10 ns ticks, filled
-     * when tuning->flags has MBE_FLAG_SECTIONS (the reference's DESECTION
-     * build, src/header.cuh:25-32): other, root build, candidate selection,
-     * L' construction, maximality check (non-maximal), check + expansion
-     * (maximal), report, re-localisation, donation, stolen-subtree fetch,
-     * idle wait. */
+     * when tuning->flags has MBE_FLAG_SECTIONS, in the slots of the
+     * reference's DESECTION build (src/header.cuh:25-32) as
+     * utils/plot_section.py:7-8 groups them: 0 other, 1 re-localisation,
+     * 2 donation (work stealing), 3 L' construction, 4 maximality check of
+     * non-maximal nodes, 5 check + expansion of maximal nodes, 6 report,
+     * 7 candidate selection, 8 root claim and localisation (subtree
+     * fetching), 9 stolen-subtree load (work stealing), 10 idle wait. */
     uint64_t sections[11];
     /* Algorithmic work: 64-bit words of the local bit matrices and set
      * bitsets that the visited nodes read (from the set sizes, not counted by

@@ -11,6 +11,7 @@
 // Options: -v variant(0..3) -w workers -l lds_words -a arena_words -s slots
 //          -c list_cutoff -r compact_ratio -i donate_interval -t twins(0|2)
 //          -b root_begin -e root_end -k root_stride -S side(-1|0|1)
+//          -p sparse_ratio (list-mode roots; 0 = never, 1 = nearly always)
 #include "wave_emu.h"
 #include "../../cumbe_amd/csrc/mbe_kernel.hip"
 
@@ -30,8 +31,9 @@ using namespace mbe;
 
 struct Opt {
     int variant = 0, side = -1;
+    uint32_t sparse = 0;
     uint32_t workers = 8, lds_words = 3072, slots = 64, cutoff = 0, ratio = 2, interval = 64,
-             twins = 2, rb = 0, re = 0, rs = 1;
+             twins = 1, rb = 0, re = 0, rs = 1;
     uint64_t arena_words = 1u << 18;
 };
 
@@ -76,6 +78,7 @@ int main(int argc, char** argv) {
             case 'e': o.re = (uint32_t)v; break;
             case 'k': o.rs = (uint32_t)v; break;
             case 'S': o.side = (int)v; break;
+            case 'p': o.sparse = (uint32_t)v; break;
             default: fprintf(stderr, "unknown option -%c\n", f); return 2;
         }
     }
@@ -156,6 +159,7 @@ int main(int argc, char** argv) {
         kp.list_cutoff = o.cutoff;
         kp.compact_ratio = o.ratio;
         kp.twins = o.twins;
+        kp.sparse_ratio = o.sparse;
         kp.abort = &abort_flag;
         kp.node_count = kp.stats + kStatNodeBudget;
         kp.overflow_list = ovf;

@@ -110,6 +110,9 @@ inline void run_wave(uint32_t block, uint32_t grid, void (*fn)(void*), void* arg
     Wave* w = new Wave();
     char* stacks = (char*)malloc(kStack * 64);
     if (!stacks) die("out of memory for lane stacks");
+    // poison the lane stacks: a local read before it is written holds
+    // nonzero garbage here, as registers do on the GPU
+    memset(stacks, 0xA5, kStack * 64);
     w->fn = fn;
     w->fnarg = arg;
     g_wave = w;

@@ -82,3 +82,22 @@ def small_suite():
         out.append((f"rand_{n_u}x{n_v}_p{p}", random_graph(n_u, n_v, p, seed)))
         seed += 1
     return out
+
+
+def wide_twins(seed=5, d=4200, n_l=6000, n_c=6):
+    """One enumerated vertex x whose neighbourhood spans d > 4096 L vertices
+    (local rows of more than 64 words), and n_c higher-degree candidates
+    whose neighbourhoods meet N(x) in large random sets, two pairs of them
+    identical (twins). Returns a CSR over U = the enumerated side."""
+    rng = np.random.default_rng(seed)
+    rows = [list(range(d))]
+    sets = []
+    for i in range(n_c):
+        if i in (1, 3):
+            sets.append(sets[-1])
+            continue
+        k = int(rng.integers(d + 100, n_l))
+        sets.append(sorted(rng.choice(n_l, size=k, replace=False).tolist()))
+    rows += sets
+    edges = [(u, v) for u, r in enumerate(rows) for v in r]
+    return csr_from_edges(len(rows), n_l, edges)

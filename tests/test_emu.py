@@ -108,3 +108,40 @@ def test_config1_sample_large_q_sets():
     r = emu("-w", 8, "-b", sub["begin"], "-k", sub["stride"], "cl", gr["n_u"], gr["n_v"], gr["m"],
             gr["gamma_u"], gr["gamma_v"], gr["max_deg"], gr["seed"])
     assert (r["count"], f"{r['hash']:#018x}") == (sub["count"], sub["hash"])
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+def test_list_mode_forced(variant):
+    """Every root in list mode (sparse ratio 1): sorted adjacency lists, the
+    list check, list-mode compaction; same tree and results as bit matrices."""
+    g = M.Graph.chung_lu(400, 300, 2500, 2.1, 2.1, 0, 11)
+    ref = O.solve(*g.csr())
+    a = emu("-v", variant, "cl", 400, 300, 2500, 2.1, 2.1, 0, 11)
+    s = emu("-p", 1, "-v", variant, "cl", 400, 300, 2500, 2.1, 2.1, 0, 11)
+    assert (s["count"], s["hash"]) == (ref.count, ref.hash)
+    assert s["nodes"] == a["nodes"]
+
+
+def test_list_mode_with_stealing():
+    """List-mode roots donate levels as compacted bit-matrix snapshots."""
+    args = (6500, 1900, 0.008, 60, 8, 40, 0.05, 3)
+    g = M.Graph.er_planted(*args)
+    ref = O.solve_roots(*g.csr(), begin=0, end=12, threads=4)
+    r = emu("-p", 1, "-w", 8, "-e", 12, "-l", 600, "-i", 8, "er", *args)
+    assert (r["count"], r["hash"]) == (ref.count, ref.hash)
+    assert r["stolen"] > 0
+
+
+def test_wide_rows_twin_pruning(tmp_path):
+    """A root whose local rows span 66 words (the wide branch of twin
+    pruning) with twin candidates: pruned, reference tree and list mode."""
+    from graphs import wide_twins
+    g = wide_twins()
+    ref = O.solve(*g)
+    path = bi_file(tmp_path, "wide", g)
+    a = emu("bi", path)
+    b = emu("-t", 0, "bi", path)
+    s = emu("-p", 1, "bi", path)
+    for r in (a, b, s):
+        assert (r["count"], r["hash"]) == (ref.count, ref.hash)
+    assert a["nodes"] < b["nodes"] and s["nodes"] == a["nodes"]

@@ -54,6 +54,46 @@ def test_cli_section_percentages(c0):
     assert len(vals) == 11 and abs(sum(vals) - 100.0) < 0.1
 
 
+# utils/plot_section.py:7-8 of the reference: the sections its plots draw and
+# the time-percentage slots each one sums
+PLOT_SECTIONS = ["L' Construction", "Maximality Checking", "Maximality Expansion",
+                 "Candidate Selection", "Subtree Fetching", "Work-stealing"]
+PLOT_INGREDIENTS = [[3], [4], [5], [7], [8], [2, 9]]
+
+
+def test_desection_alias_through_the_plot_table(c0):
+    """`mbe_desection` (the name the reference's harness runs, utils/run.py:55)
+    prints the time-percentage line without MBE_SECTIONS; mapped through the
+    plot script's ingredient table, selection time lands under Candidate
+    Selection and the solve's other sections under theirs."""
+    d, path, gold = c0
+    r = subprocess.run([os.path.join(ROOT, "cumbe_amd", "mbe_desection"), str(path), "cuMBE"],
+                       cwd=d, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    line = [x for x in r.stdout.splitlines() if x.startswith("time percentage:")]
+    assert len(line) == 1
+    vals = [float(v) for v in line[0].replace("time percentage:", "").split()]
+    plot = {s: sum(vals[i] for i in ing) for s, ing in zip(PLOT_SECTIONS, PLOT_INGREDIENTS)}
+    for s in ("L' Construction", "Maximality Checking", "Candidate Selection", "Subtree Fetching"):
+        assert plot[s] > 0, (s, vals)
+    # the same slots through the ABI: slot 7 is the selection clock
+    g = M.Graph.load_bi(str(path))
+    res = M.solve(g, "cuMBE", {"flags": M.FLAG_SECTIONS})
+    assert M.SECTION_NAMES[7] == "select" and res.sections[7] > 0
+    assert f"maximal bicliques: {gold['count']}" in r.stdout
+
+
+def test_devariance_alias(c0):
+    d, path, gold = c0
+    r = subprocess.run([os.path.join(ROOT, "cumbe_amd", "mbe_devariance"), str(path), "noWS"],
+                       cwd=d, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    out = r.stdout.splitlines()
+    workers = int(out[4].split(",")[0].split(":")[1])
+    assert len(out[5:-2]) == workers
+    assert out[-2] == f"maximal bicliques: {gold['count']}"
+
+
 def test_cli_worker_variance(c0):
     d, path, gold = c0
     out = run(d, [str(path), "noWS"], {"MBE_VARIANCE": "1"})

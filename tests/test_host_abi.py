@@ -226,3 +226,18 @@ def test_serve_bench_helpers(tmp_path):
     assert len(seeds) == 3471 and seeds[0] == 1000 and 1006 not in seeds
     light = SB.seeds_from_csv(os.path.join(ROOT, "tests", "golden", "batch_c3.csv"), max_work=10**6)
     assert set(light) < set(seeds)
+
+
+def test_section_slots_follow_the_reference_plot_table():
+    """mbe_result.sections uses the reference's DESECTION slots as its plot
+    script groups them (utils/plot_section.py:7-8): 3 L', 4 checking,
+    5 expansion, 7 selection, 8 subtree fetching, 2 + 9 work stealing."""
+    names = M.SECTION_NAMES
+    assert len(names) == 11
+    assert (names[3], names[4], names[5], names[7], names[8]) == (
+        "lprime", "check", "expand", "select", "root_build")
+    assert {names[2], names[9]} == {"donate", "fetch"}
+    for alias in ("mbe_desection", "mbe_devariance"):
+        p = os.path.join(ROOT, "cumbe_amd", alias)
+        assert os.path.exists(p) and os.path.realpath(p) == os.path.realpath(
+            os.path.join(ROOT, "cumbe_amd", "mbe"))
