@@ -90,7 +90,8 @@ def batch_leg(variant, reps=3):
     rows = c3_rows()
     gs = [M.Graph.batch_graph(s) for s, _, _ in rows]
     note(f"batch leg: plan over {len(gs)} graphs")
-    plan = M.BatchPlan(gs, variant)
+    plan = M.BatchPlan(gs, variant)  # the autotuner's batched entry, when cached
+    plan_tuning = plan.tuning
     plan.run()  # warm-up (first replay)
     best = None
     ok = True
@@ -125,6 +126,7 @@ def batch_leg(variant, reps=3):
         "per_graph_kernel_ms_sum": kms,
         "speedup": wall_p / wall_b,
         "parity": ok,
+        "batched_tuning": plan_tuning,
     }
 
 

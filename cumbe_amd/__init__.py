@@ -54,7 +54,7 @@ class Tuning(C.Structure):
         ("max_nodes", C.c_uint64),
         ("time_limit_ms", C.c_double),
         ("sparse_ratio", C.c_uint32),
-        ("reserved0", C.c_uint32),
+        ("split_nodes", C.c_uint32),
     ]
 
 
@@ -321,12 +321,24 @@ def last_worker_busy() -> List[int]:
     return list(buf)[:n]
 
 
+def batch_tuning(tuning: Optional[dict]) -> Optional[dict]:
+    """The batched path's launch parameters: the caller's, else the
+    autotuner's cached entry for the batched workload, else the library's
+    defaults (None)."""
+    if tuning is not None:
+        return tuning
+    from . import autotune
+    return autotune.cached("batch_sample")
+
+
 def batch_solve(graphs: List[Graph], variant: str = "cuMBE", tuning: Optional[dict] = None,
                 allow_partial: bool = False):
     """All graphs in one persistent launch; returns (per-graph results, total).
+    Without a tuning the autotuner's batched entry is used (batch_tuning).
 
     With a watchdog in `tuning` and allow_partial, a stopped run returns its
     partial results with tot.partial set."""
+    tuning = batch_tuning(tuning)
     n = len(graphs)
     arr = (C.c_void_p * max(n, 1))(*[g.handle.value for g in graphs])
     per = (Result * max(n, 1))()
@@ -345,6 +357,8 @@ class BatchPlan:
 
     def __init__(self, graphs: List[Graph], variant: str = "cuMBE",
                  tuning: Optional[dict] = None):
+        tuning = batch_tuning(tuning)
+        self.tuning = tuning
         self.n = len(graphs)
         arr = (C.c_void_p * max(self.n, 1))(*[g.handle.value for g in graphs])
         h = C.c_void_p()

@@ -2,17 +2,22 @@
 
 Picks, per workload key, the per-worker LDS budget together with the workers
 per CU (pairs that fill a CU's LDS), the donation interval of the
-work-stealing deque, the list-versus-bitmap cutoff (|P|+|Q| at or below which
-a node tests candidate rows instead of AND/OR-ing L' columns) and the
-re-localisation ratio. Coordinate descent from a start point: each trial is
+work-stealing deque, the two list-versus-bitmap cutoffs -- sparse_ratio (a
+root whose bit matrices would take more than this many words per local
+adjacency entry is localised as sorted lists) and list_cutoff (|P|+|Q| at or
+below which a node tests candidate rows instead of AND/OR-ing L' columns) --
+and the re-localisation ratio. Workers are single waves, so the workgroup
+size is not a search axis. Coordinate descent from a start point: each trial is
 the mean of two runs and a move needs a 5 % gain (single runs of one
 configuration vary by several percent). Every trial runs under the engine's
 watchdog (a time limit), and a trial that hits it is discarded. Results are cached in tuning_cache.json next to this file and
 returned by cached(key).
 
-Usage: python -m cumbe_amd.autotune <workload> [--budget-s S]
-  workloads: c1_roots400 (the bench workload), batch_sample (32 graphs of the
-  batched workload's distribution)
+Usage: python -m cumbe_amd.autotune <workload> [--budget-s S] [--cache PATH]
+  workloads: c1_roots400 (the bench workload), batch_sample (64 graphs of the
+  batched workload's distribution whose oracle result is known; the batched
+  path -- BatchPlan, batch_solve and the benchmark's batch leg -- launches
+  with this entry when no tuning is given)
 """
 from __future__ import annotations
 
@@ -36,8 +41,10 @@ GRID = {
     "donate_interval": [16, 64, 256, 1024],
     "list_cutoff": [0, 16, 64, 256],
     "compact_ratio": [2, 4, 0xFFFFFFFF],
+    "sparse_ratio": [8, 32, 0xFFFFFFFF],
 }
-START = {"occupancy": (3072, 6), "donate_interval": 64, "list_cutoff": 0, "compact_ratio": 2}
+START = {"occupancy": (3072, 6), "donate_interval": 64, "list_cutoff": 0, "compact_ratio": 2,
+         "sparse_ratio": 0xFFFFFFFF}
 
 
 def expand(point: dict) -> dict:
@@ -99,6 +106,20 @@ def tune(key: str, run: Callable[[dict], Optional[float]], budget_s: float,
     return expand(best)
 
 
+def batch_sample_seeds(n: int = 64) -> List[int]:
+    """Every 54th seed of the batched workload whose oracle result is known
+    (tests/golden/batch_c3.csv), n of them."""
+    path = os.path.join(os.path.dirname(HERE), "tests", "golden", "batch_c3.csv")
+    seeds = []
+    for line in open(path):
+        if line.startswith("#") or line.startswith("seed"):
+            continue
+        f = line.strip().split(",")
+        if f[4] != "-":
+            seeds.append(int(f[0]))
+    return seeds[::54][:n]
+
+
 def _workload(key: str):
     sys.path.insert(0, os.path.dirname(HERE))
     import cumbe_amd as M
@@ -112,7 +133,7 @@ def _workload(key: str):
             return None if r.partial else r.kernel_ms
         return run
     if key == "batch_sample":
-        gs = [M.Graph.batch_graph(1000 + 128 * i) for i in range(32)]
+        gs = [M.Graph.batch_graph(s) for s in batch_sample_seeds()]
 
         def run(t):
             try:
@@ -128,7 +149,14 @@ def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("workload")
     ap.add_argument("--budget-s", type=float, default=240)
+    ap.add_argument("--cache", help="cache file to update (default: the package's)")
     args = ap.parse_args()
+    if args.cache:
+        global CACHE
+        if not os.path.exists(args.cache) and os.path.exists(CACHE):
+            with open(CACHE) as f, open(args.cache, "w") as g:
+                g.write(f.read())
+        CACHE = args.cache
     best = tune(args.workload, _workload(args.workload), args.budget_s)
     print(json.dumps({"workload": args.workload, "tuning": best}))
 

@@ -105,6 +105,7 @@ Tune resolve(const mbe_tuning* t, int cus, size_t lds_per_cu) {
     r.time_limit_ms = t ? t->time_limit_ms : 0;
     r.donate_interval = (t && t->donate_interval) ? t->donate_interval : 64;
     r.sparse_ratio = (t && t->sparse_ratio) ? (t->sparse_ratio == UINT32_MAX ? 0 : t->sparse_ratio) : 8;
+    r.split_nodes = (t && t->split_nodes) ? (t->split_nodes == UINT32_MAX ? 0 : t->split_nodes) : 4096;
     return r;
 }
 
@@ -230,6 +231,7 @@ mbe_status setup_pass(Bufs& B, hipStream_t stream, unsigned long long* abort_hos
     kp.list_cutoff = tu.list_cutoff;
     kp.compact_ratio = tu.compact_ratio;
     kp.sparse_ratio = tu.sparse_ratio;
+    kp.split_nodes = tu.split_nodes;
     kp.twins = (tu.flags & MBE_FLAG_REFERENCE_TREE) ? 0 : 1;
     HIPC(hipHostGetDevicePointer((void**)&kp.abort, abort_host, 0));
     kp.node_count = kp.stats + mbe::kStatNodeBudget;
@@ -835,6 +837,7 @@ mbe_status mbe_solve_noalloc(const void* dev_graph, const mbe_packed_graph* info
         kp.list_cutoff = tu.list_cutoff;
         kp.compact_ratio = tu.compact_ratio;
         kp.sparse_ratio = tu.sparse_ratio;
+        kp.split_nodes = tu.split_nodes;
         kp.twins = (tu.flags & MBE_FLAG_REFERENCE_TREE) ? 0 : 1;
         HIPC(hipHostGetDevicePointer((void**)&kp.abort, E.abort_flag, 0));
         kp.node_count = kp.stats + mbe::kStatNodeBudget;

@@ -90,6 +90,7 @@ struct ServeCtl {                      // device memory, zeroed before launch
     unsigned long long pub_seen;       // mirrors of the host's published count,
     unsigned long long close_seen;     // close flag and abort flag, refreshed by
     unsigned long long abort_seen;     // the polling loader
+    unsigned long long open_first;     // graphs below this have every root claimed
 };
 
 struct KParams {
@@ -125,6 +126,7 @@ struct KParams {
     uint32_t n_slots;
     uint32_t donate_depth;
     uint32_t donate_interval;   // nodes between a worker's donation checks
+    uint32_t split_nodes;       // a task past this many nodes donates while the deque is short (0 = never)
     unsigned long long* dq;     // [0] head, [1] tail, [2] idle, [3] pending, [4..] seq per slot
     // host-mapped progress: 2 slots per worker (roots done, nodes), plain stores
     unsigned long long* progress;
@@ -143,6 +145,7 @@ struct KParams {
     // serving mode (serve != null): graphs is the growing table, results,
     // remaining and done are indexed by table position
     ServeCtl* serve;
+    unsigned long long* root_cursor;      // per table position: roots claimed
     const MailRec* mail;                  // host-mapped ring of mail_cap records
     uint64_t mail_cap;
     const unsigned long long* mail_pub;   // host-mapped: records published

@@ -94,6 +94,7 @@ struct Tune {
     uint64_t max_nodes;
     double time_limit_ms;
     uint32_t sparse_ratio;  // 0 = never list mode
+    uint32_t split_nodes;   // 0 = only when workers are idle
 };
 
 // What a launch needs to know about the uploaded graphs.

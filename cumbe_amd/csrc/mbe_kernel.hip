@@ -63,6 +63,8 @@ constexpr uint32_t kLdsHdr = (kLdsSec + kSecCount + 7) & ~7u;
 static_assert(kLdsHdr <= kLdsHeaderWords, "LDS header larger than the host reserves");
 constexpr uint32_t kArenaHdr = 0;                // arena words before problems
 constexpr uint32_t kLdsFramesMin = 24;           // frames an LDS tier must hold
+constexpr uint64_t kSplitQueue = 64;             // deque length below which long tasks split
+constexpr uint32_t kSparseMaxP = 8;              // list mode only for roots with |P| <= this
 
 enum Variant : int { kCuMBE = 0, kNoRS = 1, kNoES = 2, kNoWS = 3 };
 
@@ -1535,12 +1537,15 @@ __device__ bool try_donate(const KParams& kp, const Local& S, uint32_t cur, uint
 // (src/mbe_noWS.cuh:337-615), with donation to the deque in place of
 // src/mbe_cuMBE.cuh's level-1 publish/steal, and descents into children
 // whose universe shrank by compact_ratio re-localised as nested problems.
-template <int V>
+// SP: the outermost problem may be a list-mode root (its children are bit
+// matrices); without it the node loop holds no list-mode code.
+template <int V, bool SP>
 __device__ void dfs(const KParams& kp, const DevGraph& G, uint32_t g, Local& S, u64* arena,
                     u64* lds, uint32_t* premin, u64& cnt, u64& hs, u64& nodes, u64& donated,
                     u64& words, Sec& sec) {
     u64* nest = lds + kLdsNest;
     uint32_t np = 0, depth0 = 0, k = 1;
+    const u64 split_at = kp.split_nodes ? nodes + kp.split_nodes : ~0ull;
     u64 next_check = nodes + kp.donate_interval;
     u64 next_abort = nodes + kAbortEvery;
     for (;;) {
@@ -1570,14 +1575,16 @@ __device__ void dfs(const KParams& kp, const DevGraph& G, uint32_t g, Local& S, 
             continue;
         }
         // donate when workers idle and the deque holds fewer tasks than
-        // there are idle workers; at most once per donate_interval nodes
+        // there are idle workers, or when this task has run split_nodes
+        // nodes and fewer than kSplitQueue tasks wait; at most once per
+        // donate_interval nodes
         if (V != kNoWS && (k > 1 || np > 0) && nodes >= next_check) {
             next_check = nodes + kp.donate_interval;
             u64 want = 0;
             if (lane() == 0) {
                 const u64 idle = ld_rlx(kp.dq + 2);
                 const u64 queued = ld_rlx(kp.dq + 1) - ld_rlx(kp.dq);
-                want = idle > queued;
+                want = idle > queued || (nodes >= split_at && queued < kSplitQueue);
             }
             if (uni64(want) && try_donate(kp, S, k, g, arena, lds, nest, np)) {
                 donated++;
@@ -1597,7 +1604,7 @@ __device__ void dfs(const KParams& kp, const DevGraph& G, uint32_t g, Local& S, 
         sec.mark(kSecLPrime);
         const bool fwd = V == kNoRS || pc - 1 + qc <= kp.list_cutoff;
         NodeOut o;
-        if (S.sparse) {  // list-mode root problem: the same check in every variant
+        if (SP && S.sparse) {  // list-mode root problem: the same check in every variant
             u64 sc = 0;
             o = process_sparse(S, F, Fc, nl, pc > 1 && kp.twins, sc);
             words += (u64)pc * S.W + S.Wc + 2 * S.W + nl + sc + 3 * S.Wc;
@@ -1608,7 +1615,7 @@ __device__ void dfs(const KParams& kp, const DevGraph& G, uint32_t g, Local& S, 
             words += (u64)pc * S.W + S.Wc + 2 * S.W + nl +
                      (fwd ? (u64)(pc - 1 + qc) * S.W + 3 * S.Wc : (u64)S.Wc * (nl + 3));
         }
-        if (pc > 1 && kp.twins && !S.sparse) {
+        if (pc > 1 && kp.twins && !(SP && S.sparse)) {
             const u64 r = prune_twins(F, Fc + kFrameHdr, S.cols, S.rows, S.lbits, S.list, S.W,
                                       S.Wc, nl, o.tw, o.has_tw);
             words += (o.has_tw ? 0 : (u64)S.Wc * (nl + 1)) + (r >> 32) * S.W;
@@ -1771,6 +1778,15 @@ __device__ __noinline__ int build_root_sparse(const KParams& kp, const DevGraph&
     return 1;
 }
 
+// The search of a list-mode root, out of line: the bit-matrix search that
+// every other task runs keeps its registers.
+template <int V>
+__device__ __noinline__ void dfs_sparse(const KParams& kp, const DevGraph& G, uint32_t g, Local& S,
+                                        u64* arena, u64* lds, uint32_t* premin, u64& cnt, u64& hs,
+                                        u64& nodes, u64& donated, u64& words, Sec& sec) {
+    dfs<V, true>(kp, G, g, S, arena, lds, premin, cnt, hs, nodes, donated, words, sec);
+}
+
 // Localise root x (rows/cols over N(x) x distance-two candidates) and
 // process the root node. Returns 0 done, 1 DFS from frame 1 needed, -1 does
 // not fit the arena (nothing has been reported then, so a later pass with
@@ -1866,8 +1882,10 @@ __device__ int build_root(const KParams& kp, const DevGraph& G, uint32_t x, u64*
         }
         k0 += rdlane32(inc, 63);
     }
-    // list mode when the bit matrices would dwarf the local adjacency
-    if (kp.sparse_ratio &&
+    // list mode when the bit matrices would dwarf the local adjacency and
+    // the root's tree is small (at most 2^kSparseMaxP nodes): a list-mode
+    // problem is searched by its own worker
+    if (kp.sparse_ratio && kx <= kSparseMaxP &&
         (size_t)nc * W + (size_t)S.d * ((nc + 63) >> 6) > (size_t)kp.sparse_ratio * T) {
         sync_mem();
         return build_root_sparse(kp, G, x, arena, lds, cpos, S, T, nc, kx, off, cnt, hs, nodes);
@@ -2001,37 +2019,43 @@ __device__ int build_root(const KParams& kp, const DevGraph& G, uint32_t x, u64*
 constexpr uint64_t kPollTicks = 200;   // mailbox polls at most every 2 us (100 MHz clock)
 constexpr uint32_t kLoadBatch = 64;    // records one loader turn takes
 
-// Claim the next root task of the growing task space (lane 0 decides).
-// The pending count is raised only while a task is visibly there to take:
-// idle waves poll this every iteration, and a count raised on every poll
-// would almost never read zero, so the exit test would never pass.
-__device__ bool serve_claim(const KParams& kp, u64& t) {
-    u64 got = 0, task = 0;
+// Claim the next root of the oldest graph in the table with roots left
+// (lane 0 decides): one fetch-and-add on that graph's root cursor, which
+// never has to be retried under contention. A cursor that passes the
+// graph's root count marks the graph fully claimed, and open_first moves
+// past it. The pending count is raised only while a graph is open: idle
+// waves poll this every iteration, and a count raised on every poll would
+// almost never read zero, so the exit test would never pass.
+__device__ bool serve_claim(const KParams& kp, uint32_t& g, uint32_t& x) {
+    u64 got = 0, gg = 0, xx = 0;
     if (lane() == 0) {
+        ServeCtl* sc = kp.serve;
         unsigned long long* pending = kp.dq + 3;
         for (;;) {
-            const u64 avail = ld_rlx(&kp.serve->tasks_avail);
-            const u64 c = ld_rlx(kp.cursor);
-            if (c >= avail) break;
+            // the loader published each graph's descriptor and cursor before
+            // the count (release)
+            const u64 ga =
+                __hip_atomic_load(&sc->graphs_avail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+            const u64 f = ld_rlx(&sc->open_first);
+            if (f >= ga) break;
+            const u64 n = kp.graphs[f].n_roots;
             atomicAdd(pending, 1ull);
-            if (atomicCAS(kp.cursor, c, c + 1) == c) {
+            const u64 c = atomicAdd(&kp.root_cursor[f], 1ull);
+            if (c < n) {
                 got = 1;
-                task = c;
+                gg = f;
+                xx = c;
                 break;
             }
             atomicAdd(pending, ~0ull);
+            atomicMax(&sc->open_first, f + 1);
         }
     }
-    t = uni64(task);
+    g = (uint32_t)uni64(gg);
+    x = (uint32_t)uni64(xx);
     got = uni64(got);
-    if (got) fence_acq();  // the loader published the task's graph with a release
+    if (got) fence_acq();  // the loader published the graph's arrays with a release
     return got != 0;
-}
-
-__device__ __forceinline__ uint32_t serve_graphs(const KParams& kp) {
-    u64 n = 0;
-    if (lane() == 0) n = ld_rlx(&kp.serve->graphs_avail);
-    return (uint32_t)uni64(n);
 }
 
 // Loader: one wave at a time (lock), at most every kPollTicks, takes newly
@@ -2099,6 +2123,7 @@ __device__ bool serve_load(const KParams& kp) {
             G.task_base = tb;
             ((DevGraph*)kp.graphs)[g] = G;
             kp.remaining[2 * g] = G.n_roots;
+            kp.root_cursor[g] = 0;
             kp.remaining[2 * g + 1] = 0;
             kp.results[2 * g] = 0;
             kp.results[2 * g + 1] = 0;
@@ -2139,7 +2164,7 @@ __device__ bool serve_exit(const KParams& kp, u64 pend, bool stop) {
         } else if (ld_rlx(&sc->close_seen)) {  // device mirror of the host's close
             fence_acq();
             ex = ld_rlx(&sc->taken) >= ld_rlx(&sc->pub_seen) &&
-                 ld_rlx(kp.cursor) >= ld_rlx(&sc->tasks_avail);
+                 ld_rlx(&sc->open_first) >= ld_rlx(&sc->graphs_avail);
         }
     }
     return uni64(ex) != 0;
@@ -2197,9 +2222,10 @@ __global__ __launch_bounds__(64) void mbe_persistent(KParams kp) {
         if (stop) roots_left = false;
         if (roots_left) {
             u64 t = 0;
+            uint32_t g = 0, x = 0;
             bool got;
             if (kp.serve) {
-                got = serve_claim(kp, t);
+                got = serve_claim(kp, g, x);
             } else {
                 if (lane() == 0) {
                     atomicAdd(pending, 1ull);
@@ -2213,19 +2239,29 @@ __global__ __launch_bounds__(64) void mbe_persistent(KParams kp) {
                 if (idle && lane() == 0) atomicAdd(kp.dq + 2, ~0ull);
                 idle = false;
                 backoff = 1;
-                const uint64_t task = kp.task_list ? kp.task_list[t]
-                                                   : kp.task_first + t * (kp.task_step ? kp.task_step : 1);
-                const uint32_t g = find_graph(kp, kp.serve ? serve_graphs(kp) : kp.n_graphs, task);
+                uint64_t task;
+                if (!kp.serve) {
+                    task = kp.task_list ? kp.task_list[t]
+                                        : kp.task_first + t * (kp.task_step ? kp.task_step : 1);
+                    g = find_graph(kp, kp.n_graphs, task);
+                }
                 const DevGraph G = kp.graphs[g];
-                const uint32_t x = (uint32_t)(task - G.task_base);
+                if (kp.serve)
+                    task = G.task_base + x;
+                else
+                    x = (uint32_t)(task - G.task_base);
                 u64 cnt = 0, hs = 0;
                 Local S;
                 sec.mark(kSecOther);
                 const int st = build_root<V>(kp, G, x, arena, lds_mem, mark, cpos, S, cnt, hs,
                                              nodes, build);
                 sec.mark(kSecBuild);
-                if (st > 0)
-                    dfs<V>(kp, G, g, S, arena, lds_mem, premin, cnt, hs, nodes, donated, words, sec);
+                if (st > 0 && S.sparse)
+                    dfs_sparse<V>(kp, G, g, S, arena, lds_mem, premin, cnt, hs, nodes, donated, words,
+                                  sec);
+                else if (st > 0)
+                    dfs<V, false>(kp, G, g, S, arena, lds_mem, premin, cnt, hs, nodes, donated, words,
+                                  sec);
                 if (st < 0) {
                     ovf++;
                     if (lane() == 0) {
@@ -2280,8 +2316,8 @@ __global__ __launch_bounds__(64) void mbe_persistent(KParams kp) {
                     const DevGraph G = kp.graphs[g];
                     u64 cnt = 0, hs = 0;
                     stolen++;
-                    dfs<V>(kp, G, g, S, arena, lds_mem, premin, cnt, hs, nodes, donated, words,
-                           sec);
+                    dfs<V, false>(kp, G, g, S, arena, lds_mem, premin, cnt, hs, nodes, donated, words,
+                                  sec);
                     if (lane() == 0 && cnt) {
                         atomicAdd(&kp.results[2 * g], cnt);
                         atomicAdd(&kp.results[2 * g + 1], hs);
@@ -2332,8 +2368,6 @@ __global__ __launch_bounds__(64) void mbe_persistent(KParams kp) {
                 if (i != kSecWait) busy += sec.t[i];
             kp.worker_busy[wid] = busy;
         }
-    }
-    if (lane() == 0) {
     }
 }
 

@@ -147,13 +147,10 @@ struct mbe_server {
         mbe_status s = setup_pass(b, stream, abort_flag, m, tu, 0, {}, 0, tu.arena_words,
                                   tu.workers, kp);
         if (s != MBE_OK) return s;
-        // the serving kernel walks the reference tree: with twin pruning a
-        // large request served alone measured 575 ms against 310 ms without
-        // (the same graph solved alone: 50 ms against 149 ms)
-        kp.twins = 0;
         kp.graphs = gtable.as<mbe::DevGraph>();
         kp.n_graphs = 0;
         kp.remaining = rem.as<unsigned long long>();
+        kp.root_cursor = rem.as<unsigned long long>() + 2 * (size_t)cfg.table_cap;
         kp.serve = sctl.as<mbe::ServeCtl>();
         kp.mail_cap = cfg.table_cap;
         kp.data = data.as<uint64_t>();
@@ -419,7 +416,7 @@ mbe_status make_server(mbe_server* S) {
     HIPC(hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking));
     HIPC(hipEventCreateWithFlags(&S->ev_end, hipEventDisableTiming));
     HIPC(S->gtable.ensure(cap * sizeof(mbe::DevGraph)));
-    HIPC(S->rem.ensure(cap * 16));
+    HIPC(S->rem.ensure(cap * 24));  // per position: open tasks, error bits, root cursor
     HIPC(S->sctl.ensure(sizeof(mbe::ServeCtl)));
     HIPC(S->data.ensure(S->cfg.data_bytes));
     HIPC(host_mapped(S->mail, cap * sizeof(mbe::MailRec)));

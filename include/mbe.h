@@ -79,15 +79,21 @@ typedef struct {
      * result holds the partial count, hash and node total. */
     uint64_t max_nodes;
     double time_limit_ms;
-    /* List-versus-bitmap cutoff of root localisation: a root whose bit
-     * matrices would take more than sparse_ratio 64-bit words per adjacency
-     * entry of its local graph is kept as sorted adjacency lists instead
-     * (the reference's list scans, src/mbe_cuMBE.cuh:188-236,345-390): bit
-     * rows only for its P candidates, the candidates of each L vertex and
-     * the L vertices of each candidate as lists. 0 = default (8),
-     * UINT32_MAX = never. */
+    /* List-versus-bitmap cutoff of root localisation: a root with at most 8
+     * P candidates whose bit matrices would take more than sparse_ratio
+     * 64-bit words per adjacency entry of its local graph is kept as sorted
+     * adjacency lists instead (the reference's list scans,
+     * src/mbe_cuMBE.cuh:188-236,345-390): bit rows only for its P
+     * candidates, the candidates of each L vertex and the L vertices of each
+     * candidate as lists. Such a root's tree (at most 256 nodes) is searched
+     * by one worker. 0 = default (8), UINT32_MAX = never. */
     uint32_t sparse_ratio;
-    uint32_t reserved0;
+    /* A task (root or stolen subtree) that has run this many tree nodes
+     * hands part of its work to the deque at its donation checks while fewer
+     * than 64 tasks wait there, even when no worker is idle, so that heavy
+     * roots spread before the tail. 0 = default (4096), UINT32_MAX = only
+     * when workers are idle. */
+    uint32_t split_nodes;
 } mbe_tuning;
 
 typedef struct {

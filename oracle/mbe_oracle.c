@@ -343,7 +343,9 @@ static void select_candidate(worker* w, int lvl) {
 /* Twin removal of iMBEA (Zhang et al., BMC Bioinformatics 2014), which the
  * reference leaves out; used only to check the GPU engine's default pruning.
  * After node x at lvl, every v left in P with N(v) n L = L' = N(x) n L leaves
- * P: selected later it would repeat L', which x (then in Q) covers. */
+ * P: selected later it would repeat L', which x (then in Q) covers.
+ * prune == 2 applies a plausible wrong rule instead -- dropping every v whose
+ * row merely contains L' -- so that the tests can show they catch it. */
 static void drop_twins(worker* w, int lvl) {
     const problem* pb = w->pb;
     const int32_t Llp = w->L_lp[lvl], nl = w->L_lp[lvl + 1];
@@ -354,7 +356,7 @@ static void drop_twins(worker* w, int lvl) {
             const int32_t l = w->u2L[pb->adjR[e]];
             if (l < Llp) { in_l++; if (l < nl) in_lp++; }
         }
-        if (in_l == nl && in_lp == nl) {  /* swap v out of the P prefix */
+        if (in_lp == nl && (in_l == nl || pb->prune == 2)) {  /* swap v out of the P prefix */
             const int32_t last = --w->P_lp[lvl];
             const int32_t tmp = w->P[last];
             w->P[last] = v; w->P[i] = tmp;

@@ -82,7 +82,9 @@ int mbe_oracle_solve_roots(uint32_t n_u, uint32_t n_v, const uint32_t* off_u,
 
 /* RS + ES with the reference's side choice, one thread; prune = 1 adds
  * iMBEA's twin removal (not the reference's tree), which the GPU engine
- * uses by default. *nodes receives the tree nodes below the roots. */
+ * uses by default; prune = 2 applies a wrong rule on purpose (drop every
+ * candidate whose row contains L'), which the tests show they detect.
+ * *nodes receives the tree nodes below the roots. */
 int mbe_oracle_solve_twins(uint32_t n_u, uint32_t n_v, const uint32_t* off_u,
                            const uint32_t* adj_u, int prune, mbe_oracle_result* out,
                            uint64_t* nodes);

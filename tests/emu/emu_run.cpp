@@ -12,6 +12,7 @@
 //          -c list_cutoff -r compact_ratio -i donate_interval -t twins(0|2)
 //          -b root_begin -e root_end -k root_stride -S side(-1|0|1)
 //          -p sparse_ratio (list-mode roots; 0 = never, 1 = nearly always)
+//          -x split_nodes (long tasks donate while the deque is short; 0 = never)
 #include "wave_emu.h"
 #include "../../cumbe_amd/csrc/mbe_kernel.hip"
 
@@ -31,7 +32,7 @@ using namespace mbe;
 
 struct Opt {
     int variant = 0, side = -1;
-    uint32_t sparse = 0;
+    uint32_t sparse = 0, split = 0;
     uint32_t workers = 8, lds_words = 3072, slots = 64, cutoff = 0, ratio = 2, interval = 64,
              twins = 1, rb = 0, re = 0, rs = 1;
     uint64_t arena_words = 1u << 18;
@@ -40,12 +41,31 @@ struct Opt {
 template <int V>
 static void wave_fn(void* p) { mbe_persistent<V>(*(KParams*)p); }
 
+// Words of the emulated LDS past the launch's dynamic LDS hold this pattern
+// while a wave runs; a write beyond the budget changes one and is reported.
+constexpr u64 kLdsCanary = 0xA5C3F00DDEADBEEFull;
+constexpr size_t kLdsEmuWords = sizeof(lds_mem) / sizeof(u64);
+
 static void run_pass(KParams& kp, uint32_t workers) {
     void (*fn)(void*) = kp.variant == 1 ? wave_fn<1> : kp.variant == 2 ? wave_fn<2>
                       : kp.variant == 3 ? wave_fn<3> : wave_fn<0>;
+    if (kp.lds_words > kLdsEmuWords) {
+        fprintf(stderr, "emu_run: lds_words %u beyond the emulated %zu\n", kp.lds_words,
+                kLdsEmuWords);
+        exit(3);
+    }
     std::vector<std::thread> th;
     for (uint32_t w = 0; w < workers; w++)
-        th.emplace_back([&kp, w, workers, fn] { emu::run_wave(w, workers, fn, &kp); });
+        th.emplace_back([&kp, w, workers, fn] {
+            for (size_t i = kp.lds_words; i < kLdsEmuWords; i++) lds_mem[i] = kLdsCanary;
+            emu::run_wave(w, workers, fn, &kp);
+            for (size_t i = kp.lds_words; i < kLdsEmuWords; i++)
+                if (lds_mem[i] != kLdsCanary) {
+                    fprintf(stderr, "emu_run: wave %u wrote LDS word %zu beyond lds_words %u\n",
+                            w, i, kp.lds_words);
+                    abort();
+                }
+        });
     for (auto& t : th) t.join();
 }
 
@@ -79,6 +99,7 @@ int main(int argc, char** argv) {
             case 'k': o.rs = (uint32_t)v; break;
             case 'S': o.side = (int)v; break;
             case 'p': o.sparse = (uint32_t)v; break;
+            case 'x': o.split = (uint32_t)v; break;
             default: fprintf(stderr, "unknown option -%c\n", f); return 2;
         }
     }
@@ -160,6 +181,7 @@ int main(int argc, char** argv) {
         kp.compact_ratio = o.ratio;
         kp.twins = o.twins;
         kp.sparse_ratio = o.sparse;
+        kp.split_nodes = o.split;
         kp.abort = &abort_flag;
         kp.node_count = kp.stats + kStatNodeBudget;
         kp.overflow_list = ovf;

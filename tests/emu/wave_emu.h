@@ -216,6 +216,14 @@ template <class T, class U>
 inline T atomicOr(T* p, U v) { return __atomic_fetch_or(p, (T)v, __ATOMIC_SEQ_CST); }
 template <class T, class U>
 inline T atomicAnd(T* p, U v) { return __atomic_fetch_and(p, (T)v, __ATOMIC_SEQ_CST); }
+template <class T, class U>
+inline T atomicMax(T* p, U v) {
+    T c = __atomic_load_n(p, __ATOMIC_SEQ_CST);
+    while (c < (T)v &&
+           !__atomic_compare_exchange_n(p, &c, (T)v, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST)) {
+    }
+    return c;
+}
 template <class T, class U, class V>
 inline T atomicCAS(T* p, U cmp, V val) {
     T c = (T)cmp;

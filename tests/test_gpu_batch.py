@@ -61,8 +61,9 @@ def test_batch_plan_replays_match_goldens():
 
 
 def test_batch_plan_equals_per_graph_solves():
-    """Batched (shared pool) and one-launch-per-graph results are identical,
-    including graphs the oracle did not finish in its time limit."""
+    """Batched (shared pool) and one-launch-per-graph results are identical
+    and equal the oracle's on light and heavy graphs of the batched
+    workload."""
     rows = c3_rows()
     light = [r for r in rows if r[1] is not None and r[3] <= 5_000_000][:40]
     heavy = [r for r in rows if r[1] is not None and r[3] > 200_000_000][:3]
@@ -136,3 +137,26 @@ def test_root_shards_sum_to_whole_on_gpu():
     parts = [M.solve(g, "cuMBE", {"root_begin": k, "root_stride": 5}) for k in range(5)]
     assert sum(p.count for p in parts) == gold["count"]
     assert f"{sum(p.hash for p in parts) & ((1 << 64) - 1):#018x}" == gold["hash"]
+
+
+def test_graphs_the_oracle_did_not_finish():
+    """Graphs of the batched workload that the oracle did not finish in 20 s
+    (tests/golden/c3_gpu_only.json, recorded on the GPU): one batched launch
+    over the first 96 that the GPU finished, one launch per graph, and the
+    other three variants on 8 of them reproduce the recorded results."""
+    gold = json.load(open(os.path.join(GOLD, "c3_gpu_only.json")))
+    rows = gold["finished_graphs"][:96]
+    gs = [M.Graph.batch_graph(r["seed"]) for r in rows]
+    say(f"plan over {len(gs)} graphs the oracle did not finish")
+    per, tot = M.BatchPlan(gs, "cuMBE").run()
+    for r, p in zip(rows, per):
+        assert (p.count, f"{p.hash:#018x}") == (r["count"], r["hash"]), r["seed"]
+    for r, g in list(zip(rows, gs))[:24]:
+        one = M.solve(g, "cuMBE")
+        assert (one.count, f"{one.hash:#018x}") == (r["count"], r["hash"]), r["seed"]
+    for r in gold["variants_checked"][:8]:
+        g = M.Graph.batch_graph(r["seed"])
+        for v in VARIANTS[1:]:
+            say(f"seed {r['seed']} {v}")
+            x = M.solve(g, v)
+            assert (x.count, f"{x.hash:#018x}") == (r["count"], r["hash"]), (r["seed"], v)

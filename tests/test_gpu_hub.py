@@ -33,29 +33,38 @@ def configs():
 
 
 @pytest.mark.parametrize("variant", VARIANTS)
-def test_hub_core_list_mode(variant):
+def test_hub_core(variant):
+    """The 17 highest-degree roots: the first few are localised as lists
+    (huge, sparse local problems with at most 8 P candidates), the rest as
+    bit matrices."""
     c = configs()["c1"]
     g = M.Graph.chung_lu(**c["graph"])
     for sub in c["hub_subsets"]:
-        say(f"hub roots {sub['begin']}:{sub['end']} {variant} list mode")
+        say(f"hub roots {sub['begin']}:{sub['end']} {variant}")
         r = M.solve(g, variant, {"root_begin": sub["begin"], "root_end": sub["end"]})
         assert (r.count, f"{r.hash:#018x}") == (sub["count"], sub["hash"])
+        b = M.solve(g, variant, {"root_begin": sub["begin"], "root_end": sub["end"],
+                                 "sparse_ratio": BITMAP_ONLY})
+        assert (b.count, f"{b.hash:#018x}") == (sub["count"], sub["hash"])
 
 
-def test_hub_core_list_versus_bitmap():
-    """Both representations are exact on the hub roots; the kernel times of
-    the two are printed (list mode is the default for these roots)."""
-    c = configs()["c1"]
-    g = M.Graph.chung_lu(**c["graph"])
-    sub = c["hub_subsets"][0]
-    t = {"root_begin": sub["begin"], "root_end": sub["end"]}
-    lst = M.solve(g, "cuMBE", t)
-    bmp = M.solve(g, "cuMBE", dict(t, sparse_ratio=BITMAP_ONLY))
-    say(f"hub roots: list mode {lst.kernel_ms:.1f} ms ({lst.launches} launches), "
-        f"bit matrices {bmp.kernel_ms:.1f} ms ({bmp.launches} launches)")
-    for r in (lst, bmp):
-        assert (r.count, f"{r.hash:#018x}") == (sub["count"], sub["hash"])
-    assert lst.nodes == bmp.nodes  # the same tree either way
+def test_list_versus_bitmap_on_hub_roots():
+    """Roots 0-5 of the serving workload's uncapped large class (degrees up
+    to 45,910, about 95,000 candidates each, at most 5 of them in P) are
+    localised as lists by default; both forms give the oracle's result, and
+    the kernel times of the two are printed."""
+    c = configs()["serving_large_uncapped"]
+    g = M.Graph.chung_lu(**c["graph"], seed=c["hub_roots"]["seed"])
+    for sub in c["hub_roots"]["subsets"]:
+        t = {"root_begin": sub["begin"], "root_end": sub["end"]}
+        lst = M.solve(g, "cuMBE", t)
+        bmp = M.solve(g, "cuMBE", dict(t, sparse_ratio=BITMAP_ONLY))
+        say(f"uncapped roots {sub['begin']}:{sub['end']}: lists {lst.kernel_ms:.1f} ms "
+            f"({lst.launches} launches), bit matrices {bmp.kernel_ms:.1f} ms "
+            f"({bmp.launches} launches)")
+        for r in (lst, bmp):
+            assert (r.count, f"{r.hash:#018x}") == (sub["count"], sub["hash"])
+        assert lst.nodes == bmp.nodes
 
 
 def test_uncapped_large_class_sample():

@@ -152,7 +152,7 @@ def test_autotune_coordinate_descent(tmp_path, monkeypatch):
     from cumbe_amd import autotune as A
     monkeypatch.setattr(A, "CACHE", str(tmp_path / "cache.json"))
     best_point = {"occupancy": (2560, 8), "donate_interval": 16, "list_cutoff": 64,
-                  "compact_ratio": 4}
+                  "compact_ratio": 4, "sparse_ratio": 32}
 
     def cost(t):  # separable bowl with its minimum on the grid, steep enough
         # that every step towards it beats the 3% acceptance threshold
@@ -241,3 +241,16 @@ def test_section_slots_follow_the_reference_plot_table():
         p = os.path.join(ROOT, "cumbe_amd", alias)
         assert os.path.exists(p) and os.path.realpath(p) == os.path.realpath(
             os.path.join(ROOT, "cumbe_amd", "mbe"))
+
+
+def test_batched_path_reads_the_tuning_cache(tmp_path, monkeypatch):
+    """BatchPlan and batch_solve launch with the autotuner's batched entry
+    when no tuning is given, and with the caller's tuning otherwise."""
+    from cumbe_amd import autotune as A
+    monkeypatch.setattr(A, "CACHE", str(tmp_path / "cache.json"))
+    assert M.batch_tuning(None) is None
+    A._store("batch_sample", {"lds_words": 4096, "waves_per_cu": 4}, 10.0, [])
+    assert M.batch_tuning(None) == {"lds_words": 4096, "waves_per_cu": 4}
+    assert M.batch_tuning({"workers": 8}) == {"workers": 8}
+    seeds = A.batch_sample_seeds()
+    assert len(seeds) == 64 and len(set(seeds)) == 64

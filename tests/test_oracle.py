@@ -13,8 +13,8 @@ the oracle is pinned by what the mathematics fixes:
     switches (rs/es), root-parallel threads.
 It also checks iMBEA's twin removal, which the GPU engine applies by default:
 the pruned tree reports the same bicliques as the reference's tree (and as
-brute force), and a wrong rule (dropping candidates that merely contain L')
-fails these cases.
+brute force), and a wrong rule (dropping candidates that merely contain L',
+the oracle's prune mode 2) changes the bicliques on these cases.
 """
 import itertools
 
@@ -176,3 +176,14 @@ def test_twin_removal_prunes_repeated_neighbourhoods():
     tw, n_tw = O.solve_twins(*g, prune=1)
     assert (tw.count, tw.hash) == (ref.count, ref.hash)
     assert n_tw < n_ref / 2, (n_tw, n_ref)
+
+
+def test_wrong_twin_rule_is_caught():
+    """Dropping candidates whose rows merely contain L' (not equal it within
+    L) loses bicliques: the twin cases above tell the two rules apart."""
+    wrong = 0
+    for name, g in TWIN_CASES:
+        ref, _ = O.solve_twins(*g, prune=0)
+        bad, _ = O.solve_twins(*g, prune=2)
+        wrong += (bad.count, bad.hash) != (ref.count, ref.hash)
+    assert wrong >= len(TWIN_CASES) // 3, (wrong, len(TWIN_CASES))
