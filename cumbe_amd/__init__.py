@@ -390,7 +390,7 @@ class BatchPlan:
 
 class ServerConfig(C.Structure):
     _fields_ = [("table_cap", C.c_uint32), ("queue_cap", C.c_uint32), ("max_nr", C.c_uint32),
-                ("reserved", C.c_uint32), ("hbm_budget", C.c_uint64), ("data_bytes", C.c_uint64),
+                ("large_edges", C.c_uint32), ("hbm_budget", C.c_uint64), ("data_bytes", C.c_uint64),
                 ("staging_bytes", C.c_uint64), ("tuning", Tuning)]
 
 
@@ -409,9 +409,10 @@ class Server:
     """Serving loop (mbe_server_*): submit graphs, poll their results."""
 
     def __init__(self, hbm_budget: int = 0, table_cap: int = 0, data_bytes: int = 0,
-                 staging_bytes: int = 0, tuning: Optional[dict] = None):
+                 staging_bytes: int = 0, tuning: Optional[dict] = None, large_edges: int = 0):
         cfg = ServerConfig()
         cfg.hbm_budget, cfg.table_cap = hbm_budget, table_cap
+        cfg.large_edges = large_edges
         cfg.data_bytes, cfg.staging_bytes = data_bytes, staging_bytes
         cfg.tuning.side = -1
         for k, v in (tuning or {}).items():

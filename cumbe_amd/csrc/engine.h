@@ -79,6 +79,7 @@ struct MailRec {
     uint64_t words;      // packed size in u64 words
     uint64_t off[6];     // u64 offsets of offR, adjR, offL, adjL, hR, hL in the block
     uint32_t nR, nL, n_roots, r_is_u;
+    uint32_t lane;       // claim lane: 0 small, 1 large
 };
 
 struct ServeCtl {                      // device memory, zeroed before launch
@@ -90,7 +91,8 @@ struct ServeCtl {                      // device memory, zeroed before launch
     unsigned long long pub_seen;       // mirrors of the host's published count,
     unsigned long long close_seen;     // close flag and abort flag, refreshed by
     unsigned long long abort_seen;     // the polling loader
-    unsigned long long open_first;     // graphs below this have every root claimed
+    unsigned long long lane_avail[2];  // graphs appended to each claim lane
+    unsigned long long lane_first[2];  // lane entries below this have every root claimed
 };
 
 struct KParams {
@@ -146,6 +148,7 @@ struct KParams {
     // remaining and done are indexed by table position
     ServeCtl* serve;
     unsigned long long* root_cursor;      // per table position: roots claimed
+    uint32_t* lane_list;                  // 2 x mail_cap: table positions per claim lane
     const MailRec* mail;                  // host-mapped ring of mail_cap records
     uint64_t mail_cap;
     const unsigned long long* mail_pub;   // host-mapped: records published

@@ -2019,36 +2019,47 @@ __device__ int build_root(const KParams& kp, const DevGraph& G, uint32_t x, u64*
 constexpr uint64_t kPollTicks = 200;   // mailbox polls at most every 2 us (100 MHz clock)
 constexpr uint32_t kLoadBatch = 64;    // records one loader turn takes
 
-// Claim the next root of the oldest graph in the table with roots left
-// (lane 0 decides): one fetch-and-add on that graph's root cursor, which
-// never has to be retried under contention. A cursor that passes the
-// graph's root count marks the graph fully claimed, and open_first moves
-// past it. The pending count is raised only while a graph is open: idle
-// waves poll this every iteration, and a count raised on every poll would
-// almost never read zero, so the exit test would never pass.
-__device__ bool serve_claim(const KParams& kp, uint32_t& g, uint32_t& x) {
+// Wave w prefers the large claim lane when w % kLargeShare == 0, so large
+// requests keep a bounded share of the waves while small ones go first.
+constexpr uint32_t kLargeShare = 4;
+
+// Claim a root (lane 0 decides). Each claim lane lists its graphs in
+// arrival order; a wave takes the next root of the oldest graph of its
+// preferred lane with roots left, else of the other lane: one fetch-and-add
+// on that graph's root cursor, which never has to be retried under
+// contention. A cursor that passes the graph's root count marks the graph
+// fully claimed, and the lane's first entry moves past it. The pending count
+// is raised only while a graph is open: idle waves poll this every
+// iteration, and a count raised on every poll would almost never read zero,
+// so the exit test would never pass.
+__device__ bool serve_claim(const KParams& kp, uint32_t wid, uint32_t& g, uint32_t& x) {
     u64 got = 0, gg = 0, xx = 0;
     if (lane() == 0) {
         ServeCtl* sc = kp.serve;
         unsigned long long* pending = kp.dq + 3;
-        for (;;) {
-            // the loader published each graph's descriptor and cursor before
-            // the count (release)
-            const u64 ga =
-                __hip_atomic_load(&sc->graphs_avail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-            const u64 f = ld_rlx(&sc->open_first);
-            if (f >= ga) break;
-            const u64 n = kp.graphs[f].n_roots;
-            atomicAdd(pending, 1ull);
-            const u64 c = atomicAdd(&kp.root_cursor[f], 1ull);
-            if (c < n) {
-                got = 1;
-                gg = f;
-                xx = c;
-                break;
+        const uint32_t pref = wid % kLargeShare == 0 ? 1 : 0;
+        for (uint32_t i = 0; i < 2 && !got; i++) {
+            const uint32_t ln = pref ^ i;
+            for (;;) {
+                // the loader published each graph's descriptor, cursor and
+                // lane entry before the lane's count (release)
+                const u64 la = __hip_atomic_load(&sc->lane_avail[ln], __ATOMIC_ACQUIRE,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+                const u64 f = ld_rlx(&sc->lane_first[ln]);
+                if (f >= la) break;
+                const uint32_t gl = kp.lane_list[(size_t)ln * kp.mail_cap + f];
+                const u64 n = kp.graphs[gl].n_roots;
+                atomicAdd(pending, 1ull);
+                const u64 c = atomicAdd(&kp.root_cursor[gl], 1ull);
+                if (c < n) {
+                    got = 1;
+                    gg = gl;
+                    xx = c;
+                    break;
+                }
+                atomicAdd(pending, ~0ull);
+                atomicMax(&sc->lane_first[ln], f + 1);
             }
-            atomicAdd(pending, ~0ull);
-            atomicMax(&sc->open_first, f + 1);
         }
     }
     g = (uint32_t)uni64(gg);
@@ -2093,6 +2104,8 @@ __device__ bool serve_load(const KParams& kp) {
     const u64 end = min(pub, taken + kLoadBatch);
     u64 g = uni64(lane() == 0 ? ld_rlx(&sc->graphs_avail) : 0);
     u64 tb = uni64(lane() == 0 ? ld_rlx(&sc->tasks_avail) : 0);
+    u64 la0 = uni64(lane() == 0 ? ld_rlx(&sc->lane_avail[0]) : 0);
+    u64 la1 = uni64(lane() == 0 ? ld_rlx(&sc->lane_avail[1]) : 0);
     for (u64 i = taken; i < end; i++, g++) {
         const MailRec* rec = kp.mail + (i % kp.mail_cap);
         const u64 src = uni64(rec->src), dst = uni64(rec->dst), nw = uni64(rec->words);
@@ -2124,6 +2137,10 @@ __device__ bool serve_load(const KParams& kp) {
             ((DevGraph*)kp.graphs)[g] = G;
             kp.remaining[2 * g] = G.n_roots;
             kp.root_cursor[g] = 0;
+            if (rec->lane)
+                kp.lane_list[kp.mail_cap + la1++] = (uint32_t)g;
+            else
+                kp.lane_list[la0++] = (uint32_t)g;
             kp.remaining[2 * g + 1] = 0;
             kp.results[2 * g] = 0;
             kp.results[2 * g + 1] = 0;
@@ -2137,12 +2154,16 @@ __device__ bool serve_load(const KParams& kp) {
             tb += G.n_roots;
         }
         tb = uni64(tb);
+        la0 = uni64(la0);
+        la1 = uni64(la1);
     }
     wsync();
     if (lane() == 0) {
         __threadfence();
         st_rel(&sc->graphs_avail, g);
         st_rel(&sc->tasks_avail, tb);
+        st_rel(&sc->lane_avail[0], la0);
+        st_rel(&sc->lane_avail[1], la1);
         st_rel(&sc->taken, end);
         __hip_atomic_store(kp.mail_taken, end, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
         st_rel(&sc->lock, 0ull);
@@ -2163,8 +2184,12 @@ __device__ bool serve_exit(const KParams& kp, u64 pend, bool stop) {
             ex = 1;
         } else if (ld_rlx(&sc->close_seen)) {  // device mirror of the host's close
             fence_acq();
-            ex = ld_rlx(&sc->taken) >= ld_rlx(&sc->pub_seen) &&
-                 ld_rlx(&sc->open_first) >= ld_rlx(&sc->graphs_avail);
+            // the loader counts a record as taken after appending it to its
+            // lane (release)
+            const u64 tk = __hip_atomic_load(&sc->taken, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+            ex = tk >= ld_rlx(&sc->pub_seen) &&
+                 ld_rlx(&sc->lane_first[0]) >= ld_rlx(&sc->lane_avail[0]) &&
+                 ld_rlx(&sc->lane_first[1]) >= ld_rlx(&sc->lane_avail[1]);
         }
     }
     return uni64(ex) != 0;
@@ -2225,7 +2250,7 @@ __global__ __launch_bounds__(64) void mbe_persistent(KParams kp) {
             uint32_t g = 0, x = 0;
             bool got;
             if (kp.serve) {
-                got = serve_claim(kp, g, x);
+                got = serve_claim(kp, wid, g, x);
             } else {
                 if (lane() == 0) {
                     atomicAdd(pending, 1ull);

@@ -151,6 +151,7 @@ struct mbe_server {
         kp.n_graphs = 0;
         kp.remaining = rem.as<unsigned long long>();
         kp.root_cursor = rem.as<unsigned long long>() + 2 * (size_t)cfg.table_cap;
+        kp.lane_list = (uint32_t*)(rem.as<unsigned long long>() + 3 * (size_t)cfg.table_cap);
         kp.serve = sctl.as<mbe::ServeCtl>();
         kp.mail_cap = cfg.table_cap;
         kp.data = data.as<uint64_t>();
@@ -277,6 +278,7 @@ struct mbe_server {
         m.nL = p.nL;
         m.n_roots = p.n_roots;
         m.r_is_u = p.r_is_u ? 1 : 0;
+        m.lane = p.adjR.size() >= cfg.large_edges ? 1 : 0;
     }
 
     void fail_all(mbe_status s) {
@@ -416,7 +418,8 @@ mbe_status make_server(mbe_server* S) {
     HIPC(hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking));
     HIPC(hipEventCreateWithFlags(&S->ev_end, hipEventDisableTiming));
     HIPC(S->gtable.ensure(cap * sizeof(mbe::DevGraph)));
-    HIPC(S->rem.ensure(cap * 24));  // per position: open tasks, error bits, root cursor
+    // per position: open tasks, error bits, root cursor, two lane entries
+    HIPC(S->rem.ensure(cap * 32));
     HIPC(S->sctl.ensure(sizeof(mbe::ServeCtl)));
     HIPC(S->data.ensure(S->cfg.data_bytes));
     HIPC(host_mapped(S->mail, cap * sizeof(mbe::MailRec)));
@@ -441,6 +444,7 @@ mbe_status mbe_server_create(const mbe_server_config* cfg, mbe_server** out) {
     if (cfg) S->cfg = *cfg;
     if (!S->cfg.table_cap) S->cfg.table_cap = 1u << 18;
     if (!S->cfg.queue_cap) S->cfg.queue_cap = 1u << 20;
+    if (!S->cfg.large_edges) S->cfg.large_edges = 1u << 18;
     if (!S->cfg.max_nr) S->cfg.max_nr = 1u << 17;
     if (!S->cfg.hbm_budget) S->cfg.hbm_budget = (uint64_t)64 << 30;
     if (!S->cfg.data_bytes) S->cfg.data_bytes = (uint64_t)4 << 30;

@@ -282,7 +282,10 @@ typedef struct {
     uint32_t table_cap;      /* requests per kernel generation; 0 = 1 << 18 */
     uint32_t queue_cap;      /* queued (unadmitted) requests; 0 = 1 << 20 */
     uint32_t max_nr;         /* largest enumerated side the kernel serves; 0 = 1 << 17 */
-    uint32_t reserved;
+    uint32_t large_edges;    /* requests with at least this many edges take the
+                                large claim lane: waves take roots of small
+                                requests first, and one wave in four prefers the
+                                large lane; 0 = 1 << 18, UINT32_MAX = one lane */
     uint64_t hbm_budget;     /* admission budget in bytes; 0 = 64 GiB */
     uint64_t data_bytes;     /* device CSR ring; 0 = 4 GiB */
     uint64_t staging_bytes;  /* pinned staging ring; 0 = 1 GiB */

@@ -8,14 +8,23 @@
 //   emu_run [options] cl <n_u> <n_v> <m> <gamma_u> <gamma_v> <max_deg> <seed>
 //   emu_run [options] er <n_u> <n_v> <p> <n_planted> <lo> <hi> <top_frac> <seed>
 //   emu_run [options] batch <seed>
+//   emu_run [options] serve <request>...   (the serving kernel: a host thread
+//           publishes each request's packed CSR to the kernel's mailbox, then
+//           closes it; prints "req <i> count <n> hash <0x...> err <e>" per
+//           request; a request is b<seed> (a batch graph) or
+//           c<n_u>,<n_v>,<m>,<gamma_u>,<gamma_v>,<max_deg>,<seed>)
 // Options: -v variant(0..3) -w workers -l lds_words -a arena_words -s slots
 //          -c list_cutoff -r compact_ratio -i donate_interval -t twins(0|2)
 //          -b root_begin -e root_end -k root_stride -S side(-1|0|1)
 //          -p sparse_ratio (list-mode roots; 0 = never, 1 = nearly always)
 //          -x split_nodes (long tasks donate while the deque is short; 0 = never)
+//          -L large_edges (serve: requests with at least this many edges take
+//             the large claim lane) -T table_cap (serve) -u publish gap (us)
 #include "wave_emu.h"
 #include "../../cumbe_amd/csrc/mbe_kernel.hip"
 
+#include <chrono>
+#include <cstring>
 #include <string>
 #include <thread>
 #include <vector>
@@ -33,6 +42,7 @@ using namespace mbe;
 struct Opt {
     int variant = 0, side = -1;
     uint32_t sparse = 0, split = 0;
+    uint32_t large_edges = 1u << 18, table_cap = 64, gap_us = 0;
     uint32_t workers = 8, lds_words = 3072, slots = 64, cutoff = 0, ratio = 2, interval = 64,
              twins = 1, rb = 0, re = 0, rs = 1;
     uint64_t arena_words = 1u << 18;
@@ -78,6 +88,156 @@ static void* zalloc(size_t bytes) {
     return p;
 }
 
+static size_t up_slot(size_t n) { return (n + 8 + 255) & ~(size_t)255; }
+
+// The serving kernel over a list of requests, mirroring the server's device
+// set-up (cumbe_amd/csrc/serve.cpp: launch_generation, pack).
+static int serve_main(const Opt& o, int n, char** specs) {
+    std::vector<Prepared> ps(n);
+    for (int r = 0; r < n; r++) {
+        const std::string s = specs[r];
+        Graph g;
+        if (s[0] == 'b') {
+            g = gen_batch_graph(atoll(s.c_str() + 1));
+        } else if (s[0] == 'c') {
+            unsigned nu, nv, md;
+            unsigned long long m, seed;
+            double gu, gv;
+            if (sscanf(s.c_str() + 1, "%u,%u,%llu,%lf,%lf,%u,%llu", &nu, &nv, &m, &gu, &gv, &md,
+                       &seed) != 7) {
+                fprintf(stderr, "bad request %s\n", s.c_str());
+                return 2;
+            }
+            g = gen_chung_lu(nu, nv, m, gu, gv, md, seed);
+        } else {
+            fprintf(stderr, "bad request %s\n", s.c_str());
+            return 2;
+        }
+        prepare(g, o.side, ps[r]);
+    }
+    const uint32_t cap = o.table_cap;
+    if ((uint32_t)n > cap) {
+        fprintf(stderr, "more requests than table_cap\n");
+        return 2;
+    }
+    // packed CSR words per request, in the server's layout
+    std::vector<uint64_t> words(n), start(n);
+    uint64_t total = 0, max_nr = 1;
+    for (int r = 0; r < n; r++) {
+        const Prepared& p = ps[r];
+        words[r] = (up_slot(p.offR.size() * 4) + up_slot(p.adjR.size() * 4) +
+                    up_slot(p.offL.size() * 4) + up_slot(p.adjL.size() * 4) +
+                    up_slot(p.hR.size() * 8) + up_slot(p.hL.size() * 8)) / 8;
+        start[r] = total;
+        total += words[r];
+        max_nr = std::max<uint64_t>(max_nr, p.nR);
+    }
+    uint64_t* staging = (uint64_t*)zalloc(total * 8);
+    uint64_t* data = (uint64_t*)zalloc(total * 8);
+    MailRec* mail = (MailRec*)zalloc(cap * sizeof(MailRec));
+    unsigned long long* flags = (unsigned long long*)zalloc(8 * 8);  // pub, taken, close, abort
+    unsigned long long* done = (unsigned long long*)zalloc((size_t)cap * 32);
+    unsigned long long* rem = (unsigned long long*)zalloc((size_t)cap * 32);
+    DevGraph* gtable = (DevGraph*)zalloc(cap * sizeof(DevGraph));
+    ServeCtl* sctl = (ServeCtl*)zalloc(sizeof(ServeCtl));
+    const uint64_t mark_words = (max_nr + 63) / 64 + 1, cpos_len = max_nr + 1;
+    const size_t ctl_words = 2 + kStatCount + kSecCount + 2 * (size_t)cap;
+    unsigned long long* ctl = (unsigned long long*)zalloc(ctl_words * 8);
+    unsigned long long* dq = (unsigned long long*)zalloc((4 + o.slots) * 8);
+    for (uint32_t s = 0; s < o.slots; s++) dq[4 + s] = s;
+    const uint32_t workers = o.workers;
+    const uint64_t arena_words = o.arena_words;
+    const uint64_t slot_words = std::min<uint64_t>(std::max<uint64_t>(4096, arena_words), 1u << 19);
+    const uint64_t ovf_cap = 1u << 20;
+    KParams kp{};
+    kp.graphs = gtable;
+    kp.n_graphs = 0;
+    kp.variant = (uint32_t)o.variant;
+    kp.cursor = ctl;
+    kp.overflow_count = ctl + 1;
+    kp.stats = ctl + 2;
+    kp.results = ctl + 2 + kStatCount + kSecCount;
+    kp.arena = (uint64_t*)zalloc(arena_words * workers * 8);
+    kp.arena_words = arena_words;
+    kp.mark = (uint64_t*)zalloc(mark_words * workers * 8);
+    kp.mark_words = mark_words;
+    kp.cpos = (uint32_t*)zalloc(cpos_len * workers * 4);
+    kp.cpos_len = cpos_len;
+    kp.lds_words = o.lds_words;
+    kp.list_cutoff = o.cutoff;
+    kp.compact_ratio = o.ratio;
+    kp.twins = o.twins;
+    kp.sparse_ratio = o.sparse;
+    kp.split_nodes = o.split;
+    kp.abort = flags + 3;
+    kp.node_count = kp.stats + kStatNodeBudget;
+    kp.overflow_list = (uint64_t*)zalloc(ovf_cap * 8);
+    kp.overflow_cap = ovf_cap;
+    kp.slots = (uint64_t*)zalloc((size_t)o.slots * slot_words * 8);
+    kp.slot_words = slot_words;
+    kp.n_slots = o.slots;
+    kp.donate_interval = o.interval;
+    kp.dq = dq;
+    kp.progress = (unsigned long long*)zalloc(workers * 16);
+    kp.remaining = rem;
+    kp.root_cursor = rem + 2 * (size_t)cap;
+    kp.lane_list = (uint32_t*)(rem + 3 * (size_t)cap);
+    kp.serve = sctl;
+    kp.mail = mail;
+    kp.mail_cap = cap;
+    kp.mail_pub = flags;
+    kp.mail_taken = flags + 1;
+    kp.staging = staging;
+    kp.data = data;
+    kp.close = flags + 2;
+    kp.done = done;
+    std::thread host([&] {
+        for (int r = 0; r < n; r++) {
+            if (o.gap_us) std::this_thread::sleep_for(std::chrono::microseconds(o.gap_us));
+            const Prepared& p = ps[r];
+            uint64_t* base = staging + start[r];
+            uint64_t off = 0, offs[6];
+            auto put = [&](int k, const void* src, size_t bytes) {
+                offs[k] = off;
+                if (bytes) memcpy(base + off, src, bytes);
+                off += up_slot(bytes) / 8;
+            };
+            put(0, p.offR.data(), p.offR.size() * 4);
+            put(1, p.adjR.data(), p.adjR.size() * 4);
+            put(2, p.offL.data(), p.offL.size() * 4);
+            put(3, p.adjL.data(), p.adjL.size() * 4);
+            put(4, p.hR.data(), p.hR.size() * 8);
+            put(5, p.hL.data(), p.hL.size() * 8);
+            MailRec& m = mail[r];
+            m.src = start[r];
+            m.dst = start[r];
+            m.words = off;
+            for (int k = 0; k < 6; k++) m.off[k] = offs[k];
+            m.nR = p.nR;
+            m.nL = p.nL;
+            m.n_roots = p.n_roots;
+            m.r_is_u = p.r_is_u ? 1 : 0;
+            m.lane = p.adjR.size() >= o.large_edges ? 1 : 0;
+            __atomic_store_n(flags, (unsigned long long)(r + 1), __ATOMIC_RELEASE);
+        }
+        while (__atomic_load_n(flags + 1, __ATOMIC_ACQUIRE) < (unsigned long long)n)
+            std::this_thread::sleep_for(std::chrono::microseconds(50));
+        __atomic_store_n(flags + 2, 1ull, __ATOMIC_RELEASE);
+    });
+    run_pass(kp, workers);
+    host.join();
+    int bad = 0;
+    for (int r = 0; r < n; r++) {
+        const unsigned long long* d = done + 4 * r;
+        printf("req %d count %llu hash 0x%016llx err %llu done %llu\n", r, d[0], d[1], d[3], d[2]);
+        bad |= d[2] != 1 || d[3] != 0;
+    }
+    const unsigned long long* st = ctl + 2;
+    printf("nodes %llu donated %llu stolen %llu dropped %llu\n", st[kStatNodes], st[kStatDonated],
+           st[kStatStolen], st[kStatDropped]);
+    return bad ? 5 : 0;
+}
+
 int main(int argc, char** argv) {
     Opt o;
     int i = 1;
@@ -100,11 +260,15 @@ int main(int argc, char** argv) {
             case 'S': o.side = (int)v; break;
             case 'p': o.sparse = (uint32_t)v; break;
             case 'x': o.split = (uint32_t)v; break;
+            case 'L': o.large_edges = (uint32_t)v; break;
+            case 'T': o.table_cap = (uint32_t)v; break;
+            case 'u': o.gap_us = (uint32_t)v; break;
             default: fprintf(stderr, "unknown option -%c\n", f); return 2;
         }
     }
     if (i >= argc) return 2;
     const std::string kind = argv[i];
+    if (kind == "serve") return serve_main(o, argc - i - 1, argv + i + 1);
     Graph g;
     std::string err;
     if (kind == "bi" && i + 1 < argc) {

@@ -145,3 +145,29 @@ def test_wide_rows_twin_pruning(tmp_path):
     for r in (a, b, s):
         assert (r["count"], r["hash"]) == (ref.count, ref.hash)
     assert a["nodes"] < b["nodes"] and s["nodes"] == a["nodes"]
+
+
+@pytest.mark.parametrize("large_edges", [5000, 1 << 18])
+def test_serving_kernel(large_edges):
+    """The serving kernel: a host thread publishes requests to the running
+    kernel's mailbox (staggered), a scaled-down large request among small
+    ones, in two claim lanes or one; every request equals the oracle."""
+    rows = []
+    for line in open(os.path.join(GOLD, "batch_c3.csv")):
+        if line[0] in "#s":
+            continue
+        s, _, _, _, cnt, h, w = line.strip().split(",")
+        if cnt != "-" and int(w) <= 5_000_000:
+            rows.append((f"b{s}", int(cnt), int(h, 16)))
+    rows = rows[:12]
+    big = (2000, 1000, 6000, 2.0, 2.0, 64, 11)
+    ref = O.solve(*M.Graph.chung_lu(*big).csr())
+    reqs = rows[:4] + [("c" + ",".join(map(str, big)), ref.count, ref.hash)] + rows[4:]
+    args = ["-w", 8, "-l", 4096, "-x", 256, "-L", large_edges, "-u", 200, "serve"]
+    out = subprocess.run([EMU, *map(str, args), *[r[0] for r in reqs]], capture_output=True,
+                         text=True, timeout=600)
+    assert out.returncode == 0, out.stderr + out.stdout
+    got = {int(m.group(1)): (int(m.group(2)), int(m.group(3), 16))
+           for m in re.finditer(r"req (\d+) count (\d+) hash (0x[0-9a-f]+) err 0 done 1",
+                                out.stdout)}
+    assert got == {i: (c, h) for i, (_, c, h) in enumerate(reqs)}

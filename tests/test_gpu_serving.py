@@ -173,3 +173,25 @@ def test_large_class_exact_alone_and_served():
         for s, c0, h0 in small:
             r = srv.poll(t_small[s], wait_ms=120_000)
             assert r is not None and (r.count, r.hash) == (c0, h0), s
+
+
+@pytest.mark.parametrize("large_edges", [1, 0, 0xFFFFFFFF])
+def test_claim_lanes(large_edges):
+    """Small and large requests in the two claim lanes (every request large,
+    the default split, a single lane): each result equals the oracle's."""
+    import json
+    c = json.load(open(os.path.join(ROOT, "tests", "golden", "configs.json")))["serving_large"]
+    large = [(e["seed"], e["count"], int(e["hash"], 16)) for e in c["seeds"]]
+    small = golden_rows(5_000_000, 24)
+    with M.Server(large_edges=large_edges) as srv:
+        say(f"claim lanes, large_edges {large_edges}")
+        tickets = []
+        for i, (s, cnt, h) in enumerate(small):
+            if i % 12 == 0:
+                ls, lc, lh = large[(i // 12) % len(large)]
+                tickets.append((ls, lc, lh, srv.submit(M.Graph.chung_lu(**c["graph"], seed=ls))))
+            tickets.append((s, cnt, h, srv.submit(M.Graph.batch_graph(s))))
+        for s, cnt, h, t in tickets:
+            r = srv.poll(t, wait_ms=120_000)
+            assert r is not None and r.status == 0, s
+            assert (r.count, r.hash) == (cnt, h), s
