@@ -30,10 +30,11 @@ Reported per JSON line (rank 0, one GPU):
               (tests/golden/batch_c3.csv): one captured persistent launch
               (mbe_batch_plan) against one launch per graph (mbe_solve), both
               checked against the goldens
-  serving     Poisson arrivals at 200/1000/2000 requests/s through the
+  serving     Poisson arrivals at 50/200/400/1000/2000 requests/s through the
               serving loop (cumbe_amd.serve_bench), 99 % small graphs of the
-              batched distribution and 1 % large graphs; latency percentiles
-              next to the same requests served alone
+              batched distribution and 1 % large graphs (degrees capped, named
+              in `deviation`); latency percentiles next to the same requests
+              served alone
 """
 import argparse
 import json
@@ -138,7 +139,11 @@ def serving_leg(duration, rates):
     small = {s: M.Graph.batch_graph(s) for s in pick}
     large = {s: M.Graph.chung_lu(*LARGE, s) for s in LARGE_SEEDS}
     note(f"serving leg: rates {rates}, {duration} s each")
-    return SB.run(small, large, rates, duration, 0.01, progress=note)
+    out = SB.run(small, large, rates, duration, 0.01, progress=note)
+    out["deviation"] = ("large class capped at degree 128: uncapped (200,000 x 100,000, 1.5 M "
+                        "edges, exponent 2.0) its hub core is not enumerable; duration per rate "
+                        f"{duration} s")
+    return out
 
 
 def traffic_from_profiles():
@@ -151,14 +156,24 @@ def traffic_from_profiles():
 
 
 def cpu_model():
+    """The host CPU's model name as /proc/cpuinfo (and lscpu) give it, with
+    its family/model/stepping numbers, which name the part when the model
+    name is generic (as in a VM)."""
+    info = {}
     try:
         for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
+            if ":" in line:
+                k, v = (x.strip() for x in line.split(":", 1))
+                info.setdefault(k, v)
+            elif info:
+                break  # first processor only
     except OSError:
         pass
-    import platform
-    return platform.processor() or "unknown"
+    if "model name" not in info:
+        import platform
+        return platform.processor() or "unknown"
+    ids = [f"{k} {info[k]}" for k in ("cpu family", "model", "stepping") if k in info]
+    return info["model name"] + (f" ({', '.join(ids)})" if ids else "")
 
 
 def cpu_baseline(g, gpu_tuning):
@@ -220,7 +235,7 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serve-duration", type=float, default=5.0,
                     help="seconds of arrivals per rate (60 for the full workload)")
-    ap.add_argument("--serve-rates", default="50,200,1000,2000")
+    ap.add_argument("--serve-rates", default="50,200,400,1000,2000")
     ap.add_argument("--batch-only", action="store_true",
                     help="run only the batch leg and print its object (for profiling)")
     args = ap.parse_args()

@@ -1540,9 +1540,9 @@ __device__ bool try_donate(const KParams& kp, const Local& S, uint32_t cur, uint
 // SP: the outermost problem may be a list-mode root (its children are bit
 // matrices); without it the node loop holds no list-mode code.
 template <int V, bool SP>
-__device__ __forceinline__ void dfs(const KParams& kp, const DevGraph& G, uint32_t g, Local& S,
-                                    u64* arena, u64* lds, uint32_t* premin, u64& cnt, u64& hs,
-                                    u64& nodes, u64& donated, u64& words, Sec& sec) {
+__device__ void dfs(const KParams& kp, const DevGraph& G, uint32_t g, Local& S, u64* arena,
+                    u64* lds, uint32_t* premin, u64& cnt, u64& hs, u64& nodes, u64& donated,
+                    u64& words, Sec& sec) {
     u64* nest = lds + kLdsNest;
     uint32_t np = 0, depth0 = 0, k = 1;
     const u64 split_at = kp.split_nodes ? nodes + kp.split_nodes : ~0ull;
@@ -2281,21 +2281,10 @@ __global__ __launch_bounds__(64) void mbe_persistent(KParams kp) {
                 const int st = build_root<V>(kp, G, x, arena, lds_mem, mark, cpos, S, cnt, hs,
                                              nodes, build);
                 sec.mark(kSecBuild);
-                if (st > 0 && S.sparse) {
-                    // the list-mode search is not inlined: it gets copies, so
-                    // that this function's own locals stay in registers
-                    Local S1 = S;
-                    const DevGraph G1 = G;
-                    Sec sec1 = sec;
-                    u64 c1 = 0, h1 = 0, n1 = 0, d1 = 0, w1 = 0;
-                    dfs_sparse<V>(kp, G1, g, S1, arena, lds_mem, premin, c1, h1, n1, d1, w1, sec1);
-                    cnt += c1;
-                    hs += h1;
-                    nodes += n1;
-                    donated += d1;
-                    words += w1;
-                    sec = sec1;
-                } else if (st > 0)
+                if (st > 0 && S.sparse)
+                    dfs_sparse<V>(kp, G, g, S, arena, lds_mem, premin, cnt, hs, nodes, donated, words,
+                                  sec);
+                else if (st > 0)
                     dfs<V, false>(kp, G, g, S, arena, lds_mem, premin, cnt, hs, nodes, donated, words,
                                   sec);
                 if (st < 0) {

@@ -20,6 +20,7 @@
 //          -x split_nodes (long tasks donate while the deque is short; 0 = never)
 //          -L large_edges (serve: requests with at least this many edges take
 //             the large claim lane) -T table_cap (serve) -u publish gap (us)
+//          -G seed of the garbage that fills uncleared device buffers
 #include "wave_emu.h"
 #include "../../cumbe_amd/csrc/mbe_kernel.hip"
 
@@ -88,6 +89,22 @@ static void* zalloc(size_t bytes) {
     return p;
 }
 
+// Device memory the host does not clear (hipMalloc'd buffers the kernels
+// must write before they read): filled with pseudo-random words, so that a
+// read of a word never written in this run sees garbage, as on the GPU.
+static uint64_t g_fill_seed = 1;
+static void* dalloc(size_t bytes) {
+    uint64_t* p = (uint64_t*)zalloc(bytes);
+    uint64_t x = g_fill_seed * 0x9E3779B97F4A7C15ull + bytes;
+    for (size_t i = 0; i < (bytes + 7) / 8; i++) {
+        x ^= x << 13;
+        x ^= x >> 7;
+        x ^= x << 17;
+        p[i] = x;
+    }
+    return p;
+}
+
 static size_t up_slot(size_t n) { return (n + 8 + 255) & ~(size_t)255; }
 
 // The serving kernel over a list of requests, mirroring the server's device
@@ -133,12 +150,12 @@ static int serve_main(const Opt& o, int n, char** specs) {
         max_nr = std::max<uint64_t>(max_nr, p.nR);
     }
     uint64_t* staging = (uint64_t*)zalloc(total * 8);
-    uint64_t* data = (uint64_t*)zalloc(total * 8);
+    uint64_t* data = (uint64_t*)dalloc(total * 8);
     MailRec* mail = (MailRec*)zalloc(cap * sizeof(MailRec));
     unsigned long long* flags = (unsigned long long*)zalloc(8 * 8);  // pub, taken, close, abort
     unsigned long long* done = (unsigned long long*)zalloc((size_t)cap * 32);
-    unsigned long long* rem = (unsigned long long*)zalloc((size_t)cap * 32);
-    DevGraph* gtable = (DevGraph*)zalloc(cap * sizeof(DevGraph));
+    unsigned long long* rem = (unsigned long long*)dalloc((size_t)cap * 32);
+    DevGraph* gtable = (DevGraph*)dalloc(cap * sizeof(DevGraph));
     ServeCtl* sctl = (ServeCtl*)zalloc(sizeof(ServeCtl));
     const uint64_t mark_words = (max_nr + 63) / 64 + 1, cpos_len = max_nr + 1;
     const size_t ctl_words = 2 + kStatCount + kSecCount + 2 * (size_t)cap;
@@ -157,11 +174,11 @@ static int serve_main(const Opt& o, int n, char** specs) {
     kp.overflow_count = ctl + 1;
     kp.stats = ctl + 2;
     kp.results = ctl + 2 + kStatCount + kSecCount;
-    kp.arena = (uint64_t*)zalloc(arena_words * workers * 8);
+    kp.arena = (uint64_t*)dalloc(arena_words * workers * 8);
     kp.arena_words = arena_words;
     kp.mark = (uint64_t*)zalloc(mark_words * workers * 8);
     kp.mark_words = mark_words;
-    kp.cpos = (uint32_t*)zalloc(cpos_len * workers * 4);
+    kp.cpos = (uint32_t*)dalloc(cpos_len * workers * 4);
     kp.cpos_len = cpos_len;
     kp.lds_words = o.lds_words;
     kp.list_cutoff = o.cutoff;
@@ -171,9 +188,9 @@ static int serve_main(const Opt& o, int n, char** specs) {
     kp.split_nodes = o.split;
     kp.abort = flags + 3;
     kp.node_count = kp.stats + kStatNodeBudget;
-    kp.overflow_list = (uint64_t*)zalloc(ovf_cap * 8);
+    kp.overflow_list = (uint64_t*)dalloc(ovf_cap * 8);
     kp.overflow_cap = ovf_cap;
-    kp.slots = (uint64_t*)zalloc((size_t)o.slots * slot_words * 8);
+    kp.slots = (uint64_t*)dalloc((size_t)o.slots * slot_words * 8);
     kp.slot_words = slot_words;
     kp.n_slots = o.slots;
     kp.donate_interval = o.interval;
@@ -263,6 +280,7 @@ int main(int argc, char** argv) {
             case 'L': o.large_edges = (uint32_t)v; break;
             case 'T': o.table_cap = (uint32_t)v; break;
             case 'u': o.gap_us = (uint32_t)v; break;
+            case 'G': g_fill_seed = (uint64_t)v; break;
             default: fprintf(stderr, "unknown option -%c\n", f); return 2;
         }
     }
@@ -315,11 +333,11 @@ int main(int argc, char** argv) {
     while (!tasks.empty()) {
         if (launches > 6) { fprintf(stderr, "root does not fit any arena\n"); return 4; }
         const uint64_t slot_words = std::min<uint64_t>(std::max<uint64_t>(4096, arena_words), 1u << 19);
-        uint64_t* arena = (uint64_t*)zalloc(arena_words * workers * 8);
+        uint64_t* arena = (uint64_t*)dalloc(arena_words * workers * 8);
         uint64_t* mark = (uint64_t*)zalloc(mark_words * workers * 8);
-        uint32_t* cpos = (uint32_t*)zalloc(cpos_len * workers * 4);
-        uint64_t* slots = (uint64_t*)zalloc((size_t)o.slots * slot_words * 8);
-        uint64_t* ovf = (uint64_t*)zalloc(tasks.size() * 8);
+        uint32_t* cpos = (uint32_t*)dalloc(cpos_len * workers * 4);
+        uint64_t* slots = (uint64_t*)dalloc((size_t)o.slots * slot_words * 8);
+        uint64_t* ovf = (uint64_t*)dalloc(tasks.size() * 8);
         unsigned long long* progress = (unsigned long long*)zalloc(workers * 16);
         for (uint32_t s = 0; s < o.slots; s++) dq[4 + s] = s;
         dq[0] = dq[1] = dq[2] = dq[3] = 0;
