@@ -131,7 +131,7 @@ typedef struct {
  * selected is dropped from P (iMBEA's rule, which the reference leaves
  * out): each such candidate would only yield a non-maximal leaf. The
  * bicliques reported are the same either way; the tree and the time are
- * not. The serving kernel (mbe_server_*) always walks the reference tree. */
+ * not. In the serving loop, set it in mbe_server_config.tuning.flags. */
 #define MBE_FLAG_REFERENCE_TREE 4u
 
 typedef struct mbe_batch_plan mbe_batch_plan;  /* resident batch + captured launch */
