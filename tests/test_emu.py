@@ -171,3 +171,24 @@ def test_serving_kernel(large_edges):
            for m in re.finditer(r"req (\d+) count (\d+) hash (0x[0-9a-f]+) err 0 done 1",
                                 out.stdout)}
     assert got == {i: (c, h) for i, (_, c, h) in enumerate(reqs)}
+
+
+def test_random_tunings():
+    """Seeded random graphs under random launch parameters -- tiny arenas
+    (overflow passes), tiny LDS (spill), donation every node, small deques,
+    forced list mode, early splitting, every variant -- with the uncleared
+    device buffers filled with random words; each result equals the oracle."""
+    import random
+    rng = random.Random(20261203)
+    for case in range(6):
+        ga = [rng.randint(60, 400), rng.randint(30, 250), 0, round(rng.uniform(1.8, 2.5), 2),
+              round(rng.uniform(1.8, 2.5), 2), rng.choice([0, 40]), rng.randint(1, 10**6)]
+        ga[2] = rng.randint(ga[0], ga[0] * 4)
+        ref = O.solve(*M.Graph.chung_lu(*ga).csr())
+        opts = ["-v", case % 4, "-w", rng.choice([1, 3, 8]), "-l", rng.choice([420, 1024, 6144]),
+                "-a", rng.choice([2048, 16384, 1 << 18]), "-s", rng.choice([4, 64]),
+                "-c", rng.choice([0, 8]), "-r", rng.choice([0, 2, 8]), "-i", rng.choice([1, 64]),
+                "-p", rng.choice([0, 1, 8]), "-x", rng.choice([0, 16, 4096]),
+                "-G", rng.randint(1, 10**6)]
+        r = emu(*opts, "cl", *ga)
+        assert (r["count"], r["hash"]) == (ref.count, ref.hash), (case, opts, ga)
