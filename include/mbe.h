@@ -98,7 +98,10 @@ typedef struct {
 
 typedef struct {
     uint64_t count;        /* maximal bicliques */
-    uint64_t hash;         /* order-independent 64-bit hash (see DESIGN) */
+    uint64_t hash;         /* order-independent 64-bit hash: the sum mod 2^64
+                              over bicliques (A, B) of mix64(S_U ^ rotl(S_V, 29)
+                              ^ (|A| K1 + |B| K2)), S_U and S_V the sums of the
+                              file ids' terms (cumbe_amd/csrc/hash.h) */
     double kernel_ms;      /* HIP-event time of the enumeration kernels */
     double total_ms;       /* host wall time incl. preprocessing and copies */
     uint64_t nodes;        /* enumeration-tree nodes visited */
