@@ -21,9 +21,11 @@
 //          -L large_edges (serve: requests with at least this many edges take
 //             the large claim lane) -T table_cap (serve) -u publish gap (us)
 //          -G seed of the garbage that fills uncleared device buffers
+//          -P s: print the workers' progress every s seconds (stderr)
 #include "wave_emu.h"
 #include "../../cumbe_amd/csrc/mbe_kernel.hip"
 
+#include <atomic>
 #include <chrono>
 #include <cstring>
 #include <string>
@@ -57,9 +59,31 @@ static void wave_fn(void* p) { mbe_persistent<V>(*(KParams*)p); }
 constexpr u64 kLdsCanary = 0xA5C3F00DDEADBEEFull;
 constexpr size_t kLdsEmuWords = sizeof(lds_mem) / sizeof(u64);
 
+// With -P s: print the workers' finished roots and nodes (their progress
+// slots) every s seconds while a pass runs, to tell a slow run from a stuck one.
+static uint32_t g_progress_s = 0;
+
 static void run_pass(KParams& kp, uint32_t workers) {
     void (*fn)(void*) = kp.variant == 1 ? wave_fn<1> : kp.variant == 2 ? wave_fn<2>
                       : kp.variant == 3 ? wave_fn<3> : wave_fn<0>;
+    std::atomic<bool> pass_done{false};
+    std::thread mon;
+    if (g_progress_s && kp.progress)
+        mon = std::thread([&] {
+            const auto t0 = std::chrono::steady_clock::now();
+            while (!pass_done.load()) {
+                for (uint32_t i = 0; i < g_progress_s * 10 && !pass_done.load(); i++)
+                    std::this_thread::sleep_for(std::chrono::milliseconds(100));
+                if (pass_done.load()) break;
+                unsigned long long r = 0, nd = 0;
+                for (uint32_t w = 0; w < workers; w++) {
+                    r += __atomic_load_n(&kp.progress[2 * w], __ATOMIC_RELAXED);
+                    nd += __atomic_load_n(&kp.progress[2 * w + 1], __ATOMIC_RELAXED);
+                }
+                const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+                fprintf(stderr, "progress %.0f s: roots %llu nodes %llu\n", s, r, nd);
+            }
+        });
     if (kp.lds_words > kLdsEmuWords) {
         fprintf(stderr, "emu_run: lds_words %u beyond the emulated %zu\n", kp.lds_words,
                 kLdsEmuWords);
@@ -78,6 +102,8 @@ static void run_pass(KParams& kp, uint32_t workers) {
                 }
         });
     for (auto& t : th) t.join();
+    pass_done.store(true);
+    if (mon.joinable()) mon.join();
 }
 
 static void* zalloc(size_t bytes) {
@@ -281,6 +307,7 @@ int main(int argc, char** argv) {
             case 'T': o.table_cap = (uint32_t)v; break;
             case 'u': o.gap_us = (uint32_t)v; break;
             case 'G': g_fill_seed = (uint64_t)v; break;
+            case 'P': g_progress_s = (uint32_t)v; break;
             default: fprintf(stderr, "unknown option -%c\n", f); return 2;
         }
     }
